@@ -1,0 +1,166 @@
+/*
+ * cpfem.h — C ABI of the MI355X (gfx950) crystal-plasticity hot path.
+ *
+ * One shared library, libcpfem_hip.so.  Plain pointers and sizes only; every array
+ * argument documented "device" is a HIP device pointer (for instance a torch tensor's
+ * data_ptr()), "host" arrays are ordinary CPU memory.  Every entry point returns 0 on
+ * success or a CPFEM_E* code; cpfem_last_error() gives the message.  No exception crosses
+ * the ABI.  A context is not re-entrant: use one context per stream.
+ *
+ * Layouts
+ *   quadrature point  q = 8 * cell + local_q (2x2x2 Gauss points, first coordinate slowest)
+ *   per-point arrays  structure-of-arrays [component][n_qp]  (e.g. Fp_inv[9][n_qp], row-major 3x3)
+ *   dof vector        3 * node + component   (the reference's flattened sol_list)
+ *   matrix            block-ELL of 3x3 node blocks: value (s, k) of scalar row r = 3*i + c is
+ *                     mat[(3*s + k) * n_dofs + r], column 3*col[s][i] + k, s < nslot[i]
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   cpfem_local          tensor_map / update_int_vars_map, singlecrystal_copper/models_copper.py:263-269
+ *                        (local Newton :204-249, IFT tangent :251-259, P :155-162, update :164-169)
+ *   cpfem_assemble       problem.newton_update + apply_bc_vec + get_A
+ *                        (crystal_plasticity_OR_design/solver.py:387-400, :119-133, :279-304)
+ *   cpfem_cell_stress    compute_avg_stress, models_copper.py:297-319
+ *   cpfem_update_state   update_int_vars_gp, models_copper.py:273-282
+ *   cpfem_solve          jax_solve (Jacobi-BiCGSTAB, tol = atol = 1e-10), solver.py:19-48
+ *   cpfem_set_dirichlet  fe.update_Dirichlet_boundary_conditions + node/vec/vals lists
+ *                        (singlecrystal_copper.py:186; solver.py:119-153)
+ */
+#ifndef CPFEM_ABI_H_
+#define CPFEM_ABI_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  CPFEM_OK = 0,
+  CPFEM_EBADARG = 1,    /* invalid argument */
+  CPFEM_EHIP = 2,       /* HIP runtime error */
+  CPFEM_ELOCAL = 3,     /* some point hit the local iteration cap */
+  CPFEM_EKRYLOV = 4,    /* |A x - b| >= 0.1 after the Krylov solve (solver.py:45) */
+  CPFEM_ENONFINITE = 5, /* NaN / Inf in a residual */
+  CPFEM_ENOMEM = 6
+};
+
+enum { CPFEM_FP64 = 0, CPFEM_MIXED = 1 };
+
+typedef struct cpfem_ctx cpfem_ctx;
+
+typedef struct CpfemMesh {
+  int64_t n_nodes;
+  int64_t n_cells;
+  const double* points; /* host (n_nodes, 3) */
+  const int32_t* cells; /* host (n_cells, 8), VTK hexahedron node order */
+} CpfemMesh;
+
+typedef struct CpfemSlip {
+  int32_t n_slip;           /* 12 or 24 */
+  const double* directions; /* host (n_slip, 3), normalised slip directions d */
+  const double* normals;    /* host (n_slip, 3), normalised plane normals n */
+} CpfemSlip;
+
+typedef struct CpfemState { /* device, SoA, leading dimension n_qp */
+  const double* Fp_inv;     /* [9][n_qp]      inverse plastic deformation gradient */
+  const double* g;          /* [n_slip][n_qp] slip resistance */
+  const double* slip;       /* [n_slip][n_qp] accumulated slip */
+  const double* rot;        /* [9][n_qp]      crystal rotation R (lab = R crystal) */
+} CpfemState;
+
+typedef struct CpfemStateOut { /* device, same layout as CpfemState */
+  double* Fp_inv;
+  double* g;
+  double* slip;
+} CpfemStateOut;
+
+enum { CPFEM_XM = 0, CPFEM_GSS_A, CPFEM_H, CPFEM_TSAT, CPFEM_R, CPFEM_C11, CPFEM_C12, CPFEM_C44, CPFEM_NPARAM };
+
+typedef struct CpfemParams {
+  const double* ptr[CPFEM_NPARAM]; /* device; value of point q is ptr[k][q * stride[k]] */
+  int64_t stride[CPFEM_NPARAM];    /* 1 = one value per point, 0 = one value for all */
+  double gamma0;                   /* reference slip rate, `ao` = 1e-3 (models_copper.py:147) */
+  double dt;                       /* load-step time increment */
+  double tol;                      /* local Newton tolerance on |r| (1e-8, models_copper.py:212) */
+  int32_t max_sub_step;            /* backtracking trials (5, models_copper.py:231) */
+  int32_t max_iter;                /* local iteration cap; the reference has none */
+  int32_t self_block;              /* 1: q = 1 in blocks of 3 (models_copper.py:71-76);
+                                      0: q = r everywhere (calibration/case1.py:175) */
+} CpfemParams;
+
+typedef struct CpfemSolveInfo {
+  int32_t iterations; /* BiCGSTAB counter (negative = breakdown code -10/-11) */
+  int32_t outer;      /* refinement sweeps (mixed precision) */
+  double residual;    /* |A x - b| after the solve */
+  double b_norm;
+} CpfemSolveInfo;
+
+const char* cpfem_last_error(void);
+int cpfem_version(void);
+
+/* Build the context: shape gradients, JxW, block-ELL pattern, element-to-block maps. */
+int cpfem_create(const CpfemMesh* mesh, const CpfemSlip* slip, cpfem_ctx** out);
+void cpfem_destroy(cpfem_ctx* ctx);
+/* info[0..7] = n_nodes, n_cells, n_qp, n_dofs, n_slots, uniform_cells, matrix_len, n_slip */
+int cpfem_info(const cpfem_ctx* ctx, int64_t* info);
+/* host copies of the pattern: nslot (n_nodes), col (n_slots * n_nodes) */
+int cpfem_pattern(const cpfem_ctx* ctx, int32_t* nslot, int32_t* col);
+/* shape gradients (n_cells, 8, 8, 3) and JxW (n_cells, 8), host copies */
+int cpfem_geometry(const cpfem_ctx* ctx, double* shape_grads, double* jxw);
+
+/* Dirichlet dofs (host, unique) and values; later entries of the reference lists must
+   already have overwritten earlier ones on shared dofs. */
+int cpfem_set_dirichlet(cpfem_ctx* ctx, void* stream, int64_t n, const int64_t* dofs, const double* vals);
+
+/* x0 of the incremental solve (solver.py:221-227): ubar - dofs on Dirichlet dofs, 0 elsewhere. */
+int cpfem_bc_x0(cpfem_ctx* ctx, void* stream, const double* dofs, double* x0);
+
+/* Point-level maps.  u_grad [9][n_qp] (device) or, if u_grad is NULL, the gradient of the
+   nodal dofs (device, n_dofs) with n_qp = 8 * n_cells.  Outputs may be NULL:
+   P [9][n_qp], A = dP/du_grad [81][n_qp] ((ij),(kl) row-major), sigma (Cauchy) [9][n_qp],
+   out (new history), iters [n_qp]. */
+int cpfem_local(cpfem_ctx* ctx, void* stream, int64_t n_qp, const double* u_grad, const double* dofs,
+                const CpfemState* state, const CpfemParams* params, double* P, double* A, double* sigma,
+                const CpfemStateOut* out, int32_t* iters);
+
+enum { CPFEM_APPLY_BC = 1 };
+
+/* Residual (n_dofs) and, if mat != NULL, the block-ELL Jacobian.  With CPFEM_APPLY_BC in
+   flags the Dirichlet rows follow apply_bc_vec (res = u - ubar) and the row elimination of
+   get_A (zero row, unit diagonal); without it both are the raw assembled quantities.
+   iters (n_qp) optional. */
+int cpfem_assemble(cpfem_ctx* ctx, void* stream, const double* dofs, const CpfemState* state,
+                   const CpfemParams* params, double* res, double* mat, int32_t* iters, int32_t flags);
+
+/* Cell-averaged Cauchy stress (n_cells, 3, 3), device. */
+int cpfem_cell_stress(cpfem_ctx* ctx, void* stream, const double* dofs, const CpfemState* state,
+                      const CpfemParams* params, double* sigma_cells);
+
+/* History update at the converged dofs (out must not alias state). */
+int cpfem_update_state(cpfem_ctx* ctx, void* stream, const double* dofs, const CpfemState* state,
+                       const CpfemParams* params, const CpfemStateOut* out);
+
+/* Jacobi-BiCGSTAB on the block-ELL matrix.  x holds x0 on entry.  precision CPFEM_FP64:
+   the reference iteration in FP64; CPFEM_MIXED: FP32 inner BiCGSTAB with FP64 iterative
+   refinement to the same stopping threshold.  Returns CPFEM_EKRYLOV (x set to NaN) when
+   |A x - b| >= 0.1 as the reference asserts. */
+int cpfem_solve(cpfem_ctx* ctx, void* stream, const double* mat, const double* b, double* x, double tol,
+                double atol, int32_t maxiter, int32_t precond, int32_t precision, CpfemSolveInfo* info);
+
+/* y = A x on the block-ELL matrix (device). */
+int cpfem_spmv(cpfem_ctx* ctx, void* stream, const double* mat, const double* x, double* y);
+
+/* |v|_2 of a device vector of n_dofs entries, returned on the host (synchronises);
+   *point_status (optional) receives the OR of point status bits since the last call. */
+int cpfem_norm(cpfem_ctx* ctx, void* stream, const double* v, double* out, int32_t* point_status);
+
+/* y = alpha x + beta y on n_dofs entries (device). */
+int cpfem_axpby(cpfem_ctx* ctx, void* stream, double alpha, const double* x, double beta, double* y);
+
+/* CSR export of the block-ELL matrix (host outputs: indptr n_dofs+1, indices/data nnz). */
+int cpfem_to_csr(cpfem_ctx* ctx, void* stream, const double* mat, int64_t* indptr, int32_t* indices, double* data);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CPFEM_ABI_H_ */

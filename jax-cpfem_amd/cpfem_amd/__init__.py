@@ -1,0 +1,17 @@
+"""cpfem_amd — MI355X (gfx950) crystal-plasticity FE hot path behind the reference's model/solver API.
+
+    from cpfem_amd import CrystalPlasticity, solver, box_mesh, Mesh
+
+All compute runs in libcpfem_hip.so (hand-written HIP kernels, C ABI in include/cpfem.h);
+this package is the host-side mirror of the reference's Python surface.
+"""
+from .generate_mesh import Mesh, box_mesh, get_meshio_cell_type
+from .materials import (CALIB_COPPER, CALIB_STEEL304, CALIB_TANTALUM, COPPER, PRESETS, SENS_ROT, STEEL304,
+                        TANTALUM, Material)
+from .models import CrystalPlasticity
+from .problem import BlockEllMatrix, Problem
+from .solver import solver
+
+__all__ = ["CrystalPlasticity", "Problem", "solver", "Mesh", "box_mesh", "get_meshio_cell_type", "Material",
+           "BlockEllMatrix", "COPPER", "TANTALUM", "STEEL304", "CALIB_COPPER", "CALIB_TANTALUM",
+           "CALIB_STEEL304", "SENS_ROT", "PRESETS"]

@@ -1,0 +1,724 @@
+// C-ABI layer of libcpfem_hip.so: context construction on the host (hex8 geometry, block-ELL
+// pattern, element-to-block and node-to-element maps) and the entry points of cpfem.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cpfem.h"
+#include "cp_internal.hpp"
+
+using namespace cpfem;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                            \
+  do {                                                                                       \
+    hipError_t e_ = (x);                                                                     \
+    if (e_ != hipSuccess) return fail(CPFEM_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+static int dalloc(T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  HIPCHK(hipMalloc((void**)p, n * sizeof(T)));
+  return CPFEM_OK;
+}
+
+template <typename T>
+static void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+struct cpfem_ctx {
+  int64_t n_nodes = 0, n_cells = 0, n_qp = 0, n_dofs = 0;
+  int n_slip = 0, n_slots = 0, uniform = 0;
+  SlipTable<12> tab12{};
+  SlipTable<24> tab24{};
+  std::vector<int32_t> h_nslot, h_col, h_dslot;
+  std::vector<double> h_sg, h_jxw;  // full per-cell geometry (host copy)
+  // device: mesh and maps
+  int32_t* d_cells = nullptr;
+  double* d_sg = nullptr;
+  double* d_jxw = nullptr;
+  int32_t *d_col = nullptr, *d_nslot = nullptr, *d_dslot = nullptr;
+  int32_t *d_cptr = nullptr, *d_clist = nullptr;
+  int32_t *d_nptr = nullptr, *d_nlist = nullptr;
+  uint8_t* d_isbc = nullptr;
+  uint8_t* d_nobc = nullptr;  // all-zero flags (raw residual / matrix)
+  double* d_ubar = nullptr;
+  // device: element work buffers
+  double *d_wP = nullptr, *d_wA = nullptr, *d_KE = nullptr, *d_RE = nullptr, *d_sig = nullptr;
+  int32_t* d_status = nullptr;
+  // reductions
+  int nb = 0;
+  double* d_partial = nullptr;
+  double* d_scalar = nullptr;
+  uint32_t* d_counter = nullptr;
+  // Krylov
+  KrylovScalars* d_ksc = nullptr;
+  double* kv64[10] = {};
+  float* kv32[10] = {};
+  float* d_mat32 = nullptr;
+  double *d_r64 = nullptr, *d_tmp64 = nullptr;
+  float* d_b32 = nullptr;
+  // pinned host staging
+  KrylovScalars* h_ksc = nullptr;
+  double* h_dbl = nullptr;
+  int32_t* h_int = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+
+  Geom geom() const {
+    Geom G;
+    G.cells = d_cells;
+    G.sg = d_sg;
+    G.jxw = d_jxw;
+    G.n_cells = n_cells;
+    G.n_qp = n_qp;
+    G.uniform = uniform;
+    return G;
+  }
+  BellView bell() const {
+    BellView M;
+    M.col = d_col;
+    M.nslot = d_nslot;
+    M.n_nodes = n_nodes;
+    M.n_slots = n_slots;
+    return M;
+  }
+  int64_t mat_len() const { return 3LL * n_slots * n_dofs; }
+};
+
+// ------------------------------------------------------------------------- hex8 geometry
+// Trilinear shape functions on [0,1]^3 with VTK node order and 2x2x2 Gauss points,
+// first reference coordinate slowest (the element jax-fem builds through basix).
+static const double HEX_NODES[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                       {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+
+static void hex8_ref_grads(double dN[8][8][3]) {
+  const double g[2] = {0.5 - 0.5 / std::sqrt(3.0), 0.5 + 0.5 / std::sqrt(3.0)};
+  int q = 0;
+  for (int a0 = 0; a0 < 2; ++a0)
+    for (int a1 = 0; a1 < 2; ++a1)
+      for (int a2 = 0; a2 < 2; ++a2, ++q) {
+        const double xi[3] = {g[a0], g[a1], g[a2]};
+        for (int a = 0; a < 8; ++a) {
+          double f[3], df[3];
+          for (int d = 0; d < 3; ++d) {
+            const bool one = HEX_NODES[a][d] == 1.0;
+            f[d] = one ? xi[d] : 1.0 - xi[d];
+            df[d] = one ? 1.0 : -1.0;
+          }
+          dN[q][a][0] = df[0] * f[1] * f[2];
+          dN[q][a][1] = f[0] * df[1] * f[2];
+          dN[q][a][2] = f[0] * f[1] * df[2];
+        }
+      }
+}
+
+static int build_geometry(cpfem_ctx* c, const double* pts, const int32_t* cells) {
+  double dN[8][8][3];
+  hex8_ref_grads(dN);
+  const int64_t nc = c->n_cells;
+  c->h_sg.assign((size_t)nc * 8 * 8 * 3, 0.0);
+  c->h_jxw.assign((size_t)nc * 8, 0.0);
+  for (int64_t e = 0; e < nc; ++e) {
+    double X[8][3];
+    for (int a = 0; a < 8; ++a) {
+      const int64_t nd = cells[e * 8 + a];
+      if (nd < 0 || nd >= c->n_nodes) return fail(CPFEM_EBADARG, "cell node index out of range");
+      for (int d = 0; d < 3; ++d) X[a][d] = pts[nd * 3 + d];
+    }
+    for (int q = 0; q < 8; ++q) {
+      double J[9] = {0};
+      for (int a = 0; a < 8; ++a)
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) J[3 * i + j] += X[a][i] * dN[q][a][j];
+      const double det = det3(J);
+      if (!(det > 0.0)) return fail(CPFEM_EBADARG, "non-positive element Jacobian (check hex8 node order)");
+      double Ji[9];
+      inv3(J, det, Ji);
+      for (int a = 0; a < 8; ++a)
+        for (int i = 0; i < 3; ++i) {
+          double s = 0.0;
+          for (int j = 0; j < 3; ++j) s += dN[q][a][j] * Ji[3 * j + i];
+          c->h_sg[((e * 8 + q) * 8 + a) * 3 + i] = s;
+        }
+      c->h_jxw[e * 8 + q] = det * 0.125;
+    }
+  }
+  // identical cells (structured box meshes) share one table
+  double smax = 0.0;
+  for (int i = 0; i < 192; ++i) smax = std::max(smax, std::fabs(c->h_sg[i]));
+  bool uni = true;
+  for (int64_t e = 1; e < nc && uni; ++e) {
+    for (int i = 0; i < 192; ++i)
+      if (std::fabs(c->h_sg[e * 192 + i] - c->h_sg[i]) > 1e-13 * smax) { uni = false; break; }
+    for (int q = 0; q < 8 && uni; ++q)
+      if (std::fabs(c->h_jxw[e * 8 + q] - c->h_jxw[q]) > 1e-13 * std::fabs(c->h_jxw[q])) uni = false;
+  }
+  c->uniform = uni ? 1 : 0;
+  // device layout: [24][8] (uniform) or [24][n_qp]
+  const int64_t Q = c->n_qp;
+  std::vector<double> sgd, jd;
+  if (uni) {
+    sgd.resize(24 * 8);
+    for (int q = 0; q < 8; ++q)
+      for (int k = 0; k < 24; ++k) sgd[k * 8 + q] = c->h_sg[q * 24 + k];
+    jd.assign(c->h_jxw.begin(), c->h_jxw.begin() + 8);
+  } else {
+    sgd.resize((size_t)24 * Q);
+    for (int64_t qp = 0; qp < Q; ++qp)
+      for (int k = 0; k < 24; ++k) sgd[k * Q + qp] = c->h_sg[qp * 24 + k];
+    jd = c->h_jxw;
+  }
+  int rc;
+  if ((rc = dalloc(&c->d_sg, sgd.size())) || (rc = dalloc(&c->d_jxw, jd.size()))) return rc;
+  HIPCHK(hipMemcpy(c->d_sg, sgd.data(), sgd.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_jxw, jd.data(), jd.size() * sizeof(double), hipMemcpyHostToDevice));
+  return CPFEM_OK;
+}
+
+// ------------------------------------------------------------------------- pattern / maps
+static int build_maps(cpfem_ctx* c, const int32_t* cells) {
+  const int64_t nn = c->n_nodes, nc = c->n_cells;
+  // node -> (cell, local) list, ascending cell
+  std::vector<int32_t> nptr(nn + 1, 0), nlist((size_t)nc * 8);
+  for (int64_t e = 0; e < nc; ++e)
+    for (int a = 0; a < 8; ++a) nptr[cells[e * 8 + a] + 1]++;
+  for (int64_t i = 0; i < nn; ++i) nptr[i + 1] += nptr[i];
+  {
+    std::vector<int32_t> fill(nptr.begin(), nptr.end() - 1);
+    for (int64_t e = 0; e < nc; ++e)
+      for (int a = 0; a < 8; ++a) nlist[fill[cells[e * 8 + a]]++] = (int32_t)(e * 8 + a);
+  }
+  // neighbour sets
+  std::vector<std::vector<int32_t>> nb(nn);
+  int maxs = 0;
+  for (int64_t i = 0; i < nn; ++i) {
+    auto& v = nb[i];
+    for (int32_t k = nptr[i]; k < nptr[i + 1]; ++k) {
+      const int64_t e = nlist[k] / 8;
+      for (int b = 0; b < 8; ++b) v.push_back(cells[e * 8 + b]);
+    }
+    if (v.empty()) v.push_back((int32_t)i);  // isolated node: keep a diagonal block
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    maxs = std::max(maxs, (int)v.size());
+  }
+  c->n_slots = maxs;
+  c->h_nslot.assign(nn, 0);
+  c->h_col.assign((size_t)maxs * nn, 0);
+  c->h_dslot.assign(nn, 0);
+  for (int64_t i = 0; i < nn; ++i) {
+    const auto& v = nb[i];
+    c->h_nslot[i] = (int32_t)v.size();
+    for (int s = 0; s < maxs; ++s) c->h_col[(size_t)s * nn + i] = s < (int)v.size() ? v[s] : (int32_t)i;
+    c->h_dslot[i] = (int32_t)(std::lower_bound(v.begin(), v.end(), (int32_t)i) - v.begin());
+  }
+  // (node i, slot s) -> element blocks (cell*64 + a*8 + b), ascending cell
+  const int64_t nkey = (int64_t)maxs * nn;
+  std::vector<int32_t> cptr(nkey + 1, 0), clist((size_t)nc * 64);
+  auto slot_of = [&](int64_t i, int32_t j) {
+    const auto& v = nb[i];
+    return (int)(std::lower_bound(v.begin(), v.end(), j) - v.begin());
+  };
+  std::vector<int32_t> keys((size_t)nc * 64);
+  for (int64_t e = 0; e < nc; ++e)
+    for (int a = 0; a < 8; ++a)
+      for (int b = 0; b < 8; ++b) {
+        const int64_t i = cells[e * 8 + a];
+        const int s = slot_of(i, cells[e * 8 + b]);
+        const int64_t key = (int64_t)s * nn + i;
+        keys[e * 64 + a * 8 + b] = (int32_t)key;
+        cptr[key + 1]++;
+      }
+  for (int64_t k = 0; k < nkey; ++k) cptr[k + 1] += cptr[k];
+  {
+    std::vector<int32_t> fill(cptr.begin(), cptr.end() - 1);
+    for (int64_t t = 0; t < nc * 64; ++t) clist[fill[keys[t]]++] = (int32_t)t;
+  }
+  int rc;
+  if ((rc = dalloc(&c->d_nptr, nptr.size())) || (rc = dalloc(&c->d_nlist, nlist.size())) ||
+      (rc = dalloc(&c->d_cptr, cptr.size())) || (rc = dalloc(&c->d_clist, clist.size())) ||
+      (rc = dalloc(&c->d_col, c->h_col.size())) || (rc = dalloc(&c->d_nslot, c->h_nslot.size())) ||
+      (rc = dalloc(&c->d_dslot, c->h_dslot.size())))
+    return rc;
+  HIPCHK(hipMemcpy(c->d_nptr, nptr.data(), nptr.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_nlist, nlist.data(), nlist.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_cptr, cptr.data(), cptr.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_clist, clist.data(), clist.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_col, c->h_col.data(), c->h_col.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_nslot, c->h_nslot.data(), c->h_nslot.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_dslot, c->h_dslot.data(), c->h_dslot.size() * 4, hipMemcpyHostToDevice));
+  return CPFEM_OK;
+}
+
+// ------------------------------------------------------------------------- helpers
+static int check_state(const cpfem_ctx* c, const CpfemState* s, const CpfemParams* p) {
+  if (!s || !s->Fp_inv || !s->g || !s->slip || !s->rot) return fail(CPFEM_EBADARG, "state pointers must be set");
+  if (!p) return fail(CPFEM_EBADARG, "params must be set");
+  for (int k = 0; k < CPFEM_NPARAM; ++k) {
+    if (!p->ptr[k]) return fail(CPFEM_EBADARG, "material parameter pointer " + std::to_string(k) + " is null");
+    if (p->stride[k] != 0 && p->stride[k] != 1) return fail(CPFEM_EBADARG, "parameter stride must be 0 or 1");
+  }
+  if (!(p->tol > 0.0) || p->max_sub_step < 1 || p->max_iter < 1) return fail(CPFEM_EBADARG, "bad local solver settings");
+  (void)c;
+  return CPFEM_OK;
+}
+
+static StateView state_view(const CpfemState* s, int64_t ld) {
+  StateView v;
+  v.Fpi = s->Fp_inv;
+  v.g = s->g;
+  v.slip = s->slip;
+  v.rot = s->rot;
+  v.ld = ld;
+  return v;
+}
+
+static MatView mat_view(const CpfemParams* p) {
+  MatView m;
+  for (int k = 0; k < MP_COUNT; ++k) {
+    m.ptr[k] = p->ptr[k];
+    m.stride[k] = p->stride[k];
+  }
+  m.ao = p->gamma0;
+  m.dt = p->dt;
+  return m;
+}
+
+static SolveCfg solve_cfg(const CpfemParams* p) {
+  SolveCfg s;
+  s.tol = p->tol;
+  s.max_sub_step = p->max_sub_step;
+  s.max_iter = p->max_iter;
+  return s;
+}
+
+static int ensure_matrix_buffers(cpfem_ctx* c) {
+  int rc;
+  if (!c->d_wA && (rc = dalloc(&c->d_wA, (size_t)81 * c->n_qp))) return rc;
+  if (!c->d_KE && (rc = dalloc(&c->d_KE, (size_t)576 * c->n_cells))) return rc;
+  return CPFEM_OK;
+}
+
+static int ensure_krylov(cpfem_ctx* c, bool mixed) {
+  int rc;
+  const int64_t n = c->n_dofs;
+  if (!c->d_ksc) {
+    if ((rc = dalloc(&c->d_ksc, 1))) return rc;
+    for (int i = 0; i < 10; ++i)
+      if ((rc = dalloc(&c->kv64[i], n))) return rc;
+    if ((rc = dalloc(&c->d_r64, n)) || (rc = dalloc(&c->d_tmp64, n))) return rc;
+    HIPCHK(hipMemset(c->d_ksc, 0, sizeof(KrylovScalars)));
+  }
+  if (mixed && !c->d_mat32) {
+    for (int i = 0; i < 10; ++i)
+      if ((rc = dalloc(&c->kv32[i], n))) return rc;
+    if ((rc = dalloc(&c->d_mat32, c->mat_len())) || (rc = dalloc(&c->d_b32, n))) return rc;
+  }
+  return CPFEM_OK;
+}
+
+// |a . b| finished on the device, copied to the host (synchronising)
+static int dot_host(cpfem_ctx* c, hipStream_t st, const double* a, const double* b, double* out) {
+  launch_residual_norm(c->n_dofs, a, b, c->d_partial, c->nb, c->d_scalar, c->d_counter, st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(c->h_dbl, c->d_scalar, sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  *out = c->h_dbl[0];
+  return CPFEM_OK;
+}
+
+template <typename T>
+static KVec<T> kvec(T** v, T* x) {
+  KVec<T> k;
+  k.x = x;
+  k.r = v[0];
+  k.rhat = v[1];
+  k.p = v[2];
+  k.q = v[3];
+  k.phat = v[4];
+  k.s = v[5];
+  k.shat = v[6];
+  k.t = v[7];
+  k.invd = v[8];
+  return k;
+}
+
+// One BiCGSTAB run (JAX semantics) on the ctx's scalar block; returns iteration counter.
+template <typename T>
+static int run_bicgstab(cpfem_ctx* c, hipStream_t st, const T* vals, const T* b, T* x, T** vecs, double tol, double atol,
+                        int maxiter, int precond, int32_t* k_out) {
+  const BellView M = c->bell();
+  launch_inv_diag<T>(M, vals, c->d_dslot, vecs[8], precond, st);
+  KrylovScalars h{};
+  h.tol = tol;
+  h.atol = atol;
+  h.maxiter = maxiter;
+  *c->h_ksc = h;
+  HIPCHK(hipMemcpyAsync(c->d_ksc, c->h_ksc, sizeof(KrylovScalars), hipMemcpyHostToDevice, st));
+  KrylovWork w;
+  w.n = c->n_dofs;
+  w.nb = c->nb;
+  w.partial = c->d_partial;
+  w.sc = c->d_ksc;
+  KVec<T> v = kvec<T>(vecs, x);
+  krylov_init<T>(M, vals, b, v, w, tol, atol, maxiter, st);
+  HIPCHK(hipGetLastError());
+  // Batches of iterations; the done flag of batch j is read while batch j+1 runs.
+  const int CH = 16;
+  int issued = 0, batch = 0;
+  bool done = false;
+  while (!done) {
+    for (int i = 0; i < CH; ++i) krylov_iteration<T>(M, vals, v, w, st);
+    issued += CH;
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(&c->h_int[2 * (batch & 1)], &c->d_ksc->k, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipEventRecord(c->ev[batch & 1], st));
+    if (batch > 0) {
+      const int pb = (batch - 1) & 1;
+      HIPCHK(hipEventSynchronize(c->ev[pb]));
+      if (c->h_int[2 * pb + 1] != 0 || issued > maxiter + 2 * CH) done = true;
+    }
+    ++batch;
+    if (issued > maxiter + 4 * CH) break;
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipMemcpy(c->h_ksc, c->d_ksc, sizeof(KrylovScalars), hipMemcpyDeviceToHost));
+  *k_out = c->h_ksc->k;
+  return CPFEM_OK;
+}
+
+// ------------------------------------------------------------------------- C ABI
+extern "C" {
+
+const char* cpfem_last_error(void) { return g_err.c_str(); }
+int cpfem_version(void) { return 1; }
+
+int cpfem_create(const CpfemMesh* mesh, const CpfemSlip* slip, cpfem_ctx** out) {
+  if (!mesh || !slip || !out || !mesh->points || !mesh->cells) return fail(CPFEM_EBADARG, "null argument");
+  if (mesh->n_nodes <= 0 || mesh->n_cells <= 0) return fail(CPFEM_EBADARG, "empty mesh");
+  if (mesh->n_nodes > (1LL << 30) || mesh->n_cells * 64 > (1LL << 31) - 1) return fail(CPFEM_EBADARG, "mesh too large for int32 maps");
+  if (slip->n_slip != 12 && slip->n_slip != 24) return fail(CPFEM_EBADARG, "n_slip must be 12 or 24");
+  if (!slip->directions || !slip->normals) return fail(CPFEM_EBADARG, "slip table pointers are null");
+  cpfem_ctx* c = new cpfem_ctx();
+  c->n_nodes = mesh->n_nodes;
+  c->n_cells = mesh->n_cells;
+  c->n_qp = 8 * mesh->n_cells;
+  c->n_dofs = 3 * mesh->n_nodes;
+  c->n_slip = slip->n_slip;
+  for (int a = 0; a < slip->n_slip; ++a)
+    for (int d = 0; d < 3; ++d) {
+      if (slip->n_slip == 12) {
+        c->tab12.d[a][d] = slip->directions[3 * a + d];
+        c->tab12.n[a][d] = slip->normals[3 * a + d];
+      } else {
+        c->tab24.d[a][d] = slip->directions[3 * a + d];
+        c->tab24.n[a][d] = slip->normals[3 * a + d];
+      }
+    }
+  int rc;
+  if ((rc = dalloc(&c->d_cells, (size_t)8 * c->n_cells))) { cpfem_destroy(c); return rc; }
+  if (hipMemcpy(c->d_cells, mesh->cells, (size_t)32 * c->n_cells, hipMemcpyHostToDevice) != hipSuccess) {
+    cpfem_destroy(c);
+    return fail(CPFEM_EHIP, "hipMemcpy cells");
+  }
+  if ((rc = build_geometry(c, mesh->points, mesh->cells)) || (rc = build_maps(c, mesh->cells))) {
+    const std::string e = g_err;
+    cpfem_destroy(c);
+    g_err = e;
+    return rc;
+  }
+  c->nb = (int)((c->n_dofs + 255) / 256);
+  const int nbq = (int)((c->n_qp + 255) / 256);
+  c->nb = std::max(c->nb, 1);
+  if ((rc = dalloc(&c->d_isbc, c->n_dofs)) || (rc = dalloc(&c->d_nobc, c->n_dofs)) || (rc = dalloc(&c->d_ubar, c->n_dofs)) ||
+      (rc = dalloc(&c->d_wP, (size_t)9 * c->n_qp)) || (rc = dalloc(&c->d_RE, (size_t)24 * c->n_cells)) ||
+      (rc = dalloc(&c->d_sig, (size_t)9 * c->n_qp)) || (rc = dalloc(&c->d_status, 1)) ||
+      (rc = dalloc(&c->d_partial, (size_t)4 * std::max(c->nb, nbq))) || (rc = dalloc(&c->d_scalar, 4)) ||
+      (rc = dalloc(&c->d_counter, 4))) {
+    const std::string e = g_err;
+    cpfem_destroy(c);
+    g_err = e;
+    return rc;
+  }
+  if (hipMemset(c->d_isbc, 0, c->n_dofs) != hipSuccess || hipMemset(c->d_nobc, 0, c->n_dofs) != hipSuccess || hipMemset(c->d_ubar, 0, 8 * c->n_dofs) != hipSuccess ||
+      hipMemset(c->d_status, 0, 4) != hipSuccess || hipMemset(c->d_counter, 0, 16) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_ksc, sizeof(KrylovScalars)) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_dbl, 8 * sizeof(double)) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_int, 8 * sizeof(int32_t)) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev[1], hipEventDisableTiming) != hipSuccess) {
+    cpfem_destroy(c);
+    return fail(CPFEM_EHIP, "context allocation failed");
+  }
+  *out = c;
+  return CPFEM_OK;
+}
+
+void cpfem_destroy(cpfem_ctx* c) {
+  if (!c) return;
+  (void)hipDeviceSynchronize();
+  dfree(c->d_cells); dfree(c->d_sg); dfree(c->d_jxw);
+  dfree(c->d_col); dfree(c->d_nslot); dfree(c->d_dslot);
+  dfree(c->d_cptr); dfree(c->d_clist); dfree(c->d_nptr); dfree(c->d_nlist);
+  dfree(c->d_isbc); dfree(c->d_nobc); dfree(c->d_ubar);
+  dfree(c->d_wP); dfree(c->d_wA); dfree(c->d_KE); dfree(c->d_RE); dfree(c->d_sig); dfree(c->d_status);
+  dfree(c->d_partial); dfree(c->d_scalar); dfree(c->d_counter);
+  dfree(c->d_ksc);
+  for (int i = 0; i < 10; ++i) { dfree(c->kv64[i]); dfree(c->kv32[i]); }
+  dfree(c->d_mat32); dfree(c->d_r64); dfree(c->d_tmp64); dfree(c->d_b32);
+  if (c->h_ksc) (void)hipHostFree(c->h_ksc);
+  if (c->h_dbl) (void)hipHostFree(c->h_dbl);
+  if (c->h_int) (void)hipHostFree(c->h_int);
+  for (int i = 0; i < 2; ++i)
+    if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+  delete c;
+}
+
+int cpfem_info(const cpfem_ctx* c, int64_t* info) {
+  if (!c || !info) return fail(CPFEM_EBADARG, "null argument");
+  info[0] = c->n_nodes;
+  info[1] = c->n_cells;
+  info[2] = c->n_qp;
+  info[3] = c->n_dofs;
+  info[4] = c->n_slots;
+  info[5] = c->uniform;
+  info[6] = c->mat_len();
+  info[7] = c->n_slip;
+  return CPFEM_OK;
+}
+
+int cpfem_pattern(const cpfem_ctx* c, int32_t* nslot, int32_t* col) {
+  if (!c || !nslot || !col) return fail(CPFEM_EBADARG, "null argument");
+  std::memcpy(nslot, c->h_nslot.data(), c->h_nslot.size() * 4);
+  std::memcpy(col, c->h_col.data(), c->h_col.size() * 4);
+  return CPFEM_OK;
+}
+
+int cpfem_geometry(const cpfem_ctx* c, double* sg, double* jxw) {
+  if (!c || !sg || !jxw) return fail(CPFEM_EBADARG, "null argument");
+  std::memcpy(sg, c->h_sg.data(), c->h_sg.size() * 8);
+  std::memcpy(jxw, c->h_jxw.data(), c->h_jxw.size() * 8);
+  return CPFEM_OK;
+}
+
+int cpfem_set_dirichlet(cpfem_ctx* c, void* stream, int64_t n, const int64_t* dofs, const double* vals) {
+  if (!c || (n > 0 && (!dofs || !vals))) return fail(CPFEM_EBADARG, "null argument");
+  std::vector<uint8_t> isbc(c->n_dofs, 0);
+  std::vector<double> ubar(c->n_dofs, 0.0);
+  for (int64_t k = 0; k < n; ++k) {
+    if (dofs[k] < 0 || dofs[k] >= c->n_dofs) return fail(CPFEM_EBADARG, "Dirichlet dof out of range");
+    isbc[dofs[k]] = 1;
+    ubar[dofs[k]] = vals[k];
+  }
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(hipMemcpyAsync(c->d_isbc, isbc.data(), c->n_dofs, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->d_ubar, ubar.data(), 8 * c->n_dofs, hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return CPFEM_OK;
+}
+
+int cpfem_local(cpfem_ctx* c, void* stream, int64_t n_qp, const double* u_grad, const double* dofs, const CpfemState* state,
+                const CpfemParams* params, double* P, double* A, double* sigma, const CpfemStateOut* out, int32_t* iters) {
+  if (!c) return fail(CPFEM_EBADARG, "null context");
+  int rc;
+  if ((rc = check_state(c, state, params))) return rc;
+  if (!u_grad) {
+    if (!dofs) return fail(CPFEM_EBADARG, "either u_grad or dofs is required");
+    if (n_qp != c->n_qp) return fail(CPFEM_EBADARG, "n_qp must equal 8 * n_cells when u_grad comes from dofs");
+  }
+  if (n_qp <= 0) return CPFEM_OK;
+  StateOut so{nullptr, nullptr, nullptr};
+  if (out) {
+    if (!out->Fp_inv || !out->g || !out->slip) return fail(CPFEM_EBADARG, "state_out pointers must all be set");
+    so.Fpi = out->Fp_inv;
+    so.g = out->g;
+    so.slip = out->slip;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const Geom G = c->geom();
+  const StateView sv = state_view(state, n_qp);
+  const MatView mv = mat_view(params);
+  const SolveCfg cfg = solve_cfg(params);
+  if (c->n_slip == 12)
+    launch_local<12>(n_qp, u_grad, G, dofs, sv, mv, c->tab12, cfg, params->self_block, P, A, sigma, so, iters, c->d_status, st);
+  else
+    launch_local<24>(n_qp, u_grad, G, dofs, sv, mv, c->tab24, cfg, params->self_block, P, A, sigma, so, iters, c->d_status, st);
+  HIPCHK(hipGetLastError());
+  return CPFEM_OK;
+}
+
+int cpfem_assemble(cpfem_ctx* c, void* stream, const double* dofs, const CpfemState* state, const CpfemParams* params,
+                   double* res, double* mat, int32_t* iters, int32_t flags) {
+  if (!c || !dofs || !res) return fail(CPFEM_EBADARG, "null argument");
+  int rc;
+  if ((rc = check_state(c, state, params))) return rc;
+  if (mat && (rc = ensure_matrix_buffers(c))) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const Geom G = c->geom();
+  const StateView sv = state_view(state, c->n_qp);
+  const MatView mv = mat_view(params);
+  const SolveCfg cfg = solve_cfg(params);
+  double* wA = mat ? c->d_wA : nullptr;
+  if (c->n_slip == 12)
+    launch_element_qp<12>(G, dofs, sv, mv, c->tab12, cfg, c->d_wP, wA, iters, c->d_status, st);
+  else
+    launch_element_qp<24>(G, dofs, sv, mv, c->tab24, cfg, c->d_wP, wA, iters, c->d_status, st);
+  launch_element_K(G, c->d_wP, wA, c->d_KE, c->d_RE, st);
+  const uint8_t* isbc = (flags & CPFEM_APPLY_BC) ? c->d_isbc : c->d_nobc;
+  launch_gather_residual(c->n_nodes, c->d_RE, c->d_nptr, c->d_nlist, isbc, c->d_ubar, dofs, res, st);
+  if (mat) launch_gather_matrix(c->bell(), c->d_KE, c->d_cptr, c->d_clist, isbc, mat, st);
+  HIPCHK(hipGetLastError());
+  return CPFEM_OK;
+}
+
+int cpfem_cell_stress(cpfem_ctx* c, void* stream, const double* dofs, const CpfemState* state, const CpfemParams* params,
+                      double* sigma_cells) {
+  if (!c || !dofs || !sigma_cells) return fail(CPFEM_EBADARG, "null argument");
+  int rc;
+  if ((rc = cpfem_local(c, stream, c->n_qp, nullptr, dofs, state, params, nullptr, nullptr, c->d_sig, nullptr, nullptr)))
+    return rc;
+  launch_cell_average(c->geom(), c->d_sig, sigma_cells, (hipStream_t)stream);
+  HIPCHK(hipGetLastError());
+  return CPFEM_OK;
+}
+
+int cpfem_update_state(cpfem_ctx* c, void* stream, const double* dofs, const CpfemState* state, const CpfemParams* params,
+                       const CpfemStateOut* out) {
+  if (!c || !dofs || !out) return fail(CPFEM_EBADARG, "null argument");
+  if (out->Fp_inv == state->Fp_inv || out->g == state->g || out->slip == state->slip)
+    return fail(CPFEM_EBADARG, "state_out must not alias state");
+  return cpfem_local(c, stream, c->n_qp, nullptr, dofs, state, params, nullptr, nullptr, nullptr, out, nullptr);
+}
+
+int cpfem_spmv(cpfem_ctx* c, void* stream, const double* mat, const double* x, double* y) {
+  if (!c || !mat || !x || !y) return fail(CPFEM_EBADARG, "null argument");
+  launch_spmv<double>(c->bell(), mat, x, y, (hipStream_t)stream);
+  HIPCHK(hipGetLastError());
+  return CPFEM_OK;
+}
+
+int cpfem_norm(cpfem_ctx* c, void* stream, const double* v, double* out, int32_t* point_status) {
+  if (!c || !v || !out) return fail(CPFEM_EBADARG, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  double s;
+  int rc;
+  if ((rc = dot_host(c, st, v, v, &s))) return rc;
+  *out = std::sqrt(s);
+  if (point_status) {
+    HIPCHK(hipMemcpy(c->h_int, c->d_status, 4, hipMemcpyDeviceToHost));
+    *point_status = c->h_int[0];
+    HIPCHK(hipMemset(c->d_status, 0, 4));
+  }
+  return CPFEM_OK;
+}
+
+int cpfem_axpby(cpfem_ctx* c, void* stream, double alpha, const double* x, double beta, double* y) {
+  if (!c || !x || !y) return fail(CPFEM_EBADARG, "null argument");
+  launch_axpby(c->n_dofs, alpha, x, beta, y, (hipStream_t)stream);
+  HIPCHK(hipGetLastError());
+  return CPFEM_OK;
+}
+
+int cpfem_solve(cpfem_ctx* c, void* stream, const double* mat, const double* b, double* x, double tol, double atol,
+                int32_t maxiter, int32_t precond, int32_t precision, CpfemSolveInfo* info) {
+  if (!c || !mat || !b || !x) return fail(CPFEM_EBADARG, "null argument");
+  if (precision != CPFEM_FP64 && precision != CPFEM_MIXED) return fail(CPFEM_EBADARG, "unknown precision");
+  if (maxiter < 0) return fail(CPFEM_EBADARG, "maxiter must be >= 0");
+  int rc;
+  if ((rc = ensure_krylov(c, precision == CPFEM_MIXED))) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t n = c->n_dofs;
+  CpfemSolveInfo inf{};
+  double bb;
+  if ((rc = dot_host(c, st, b, b, &bb))) return rc;
+  inf.b_norm = std::sqrt(bb);
+  if (precision == CPFEM_FP64) {
+    int32_t k = 0;
+    if ((rc = run_bicgstab<double>(c, st, mat, b, x, c->kv64, tol, atol, maxiter, precond, &k))) return rc;
+    inf.iterations = k;
+    inf.outer = 1;
+  } else {
+    // FP32 inner BiCGSTAB on a rounded copy of A, FP64 iterative refinement on the true
+    // residual until |b - A x| <= max(tol |b|, atol).
+    const BellView M = c->bell();
+    launch_cast_f32(c->mat_len(), mat, c->d_mat32, st);
+    const double thr2 = std::max(tol * tol * bb, atol * atol);
+    int total = 0;
+    int outer = 0;
+    for (; outer < 30; ++outer) {
+      launch_residual64(M, mat, x, b, c->d_r64, st);
+      double rr;
+      if ((rc = dot_host(c, st, c->d_r64, c->d_r64, &rr))) return rc;
+      if (!(rr > thr2) || total >= maxiter) break;
+      launch_cast_f32(n, c->d_r64, c->d_b32, st);
+      HIPCHK(hipMemsetAsync(c->kv32[9], 0, n * sizeof(float), st));
+      int32_t k = 0;
+      if ((rc = run_bicgstab<float>(c, st, c->d_mat32, c->d_b32, c->kv32[9], c->kv32, 1e-5, 0.0, maxiter - total, precond, &k)))
+        return rc;
+      total += (k > 0 ? k : 0);
+      launch_f32_add(n, c->kv32[9], x, st);
+      if (k <= 0) { ++outer; launch_residual64(M, mat, x, b, c->d_r64, st); break; }
+    }
+    inf.iterations = total;
+    inf.outer = outer;
+  }
+  // |A x - b| < 0.1 check of jax_solve (solver.py:43-46)
+  launch_residual64(c->bell(), mat, x, b, c->d_tmp64, st);
+  double err2;
+  if ((rc = dot_host(c, st, c->d_tmp64, c->d_tmp64, &err2))) return rc;
+  inf.residual = std::sqrt(err2);
+  if (info) *info = inf;
+  if (!(inf.residual < 0.1)) {
+    std::vector<double> nanv(n, std::nan(""));
+    HIPCHK(hipMemcpyAsync(x, nanv.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return fail(CPFEM_EKRYLOV, "JAX linear solver failed to converge with err = " + std::to_string(inf.residual));
+  }
+  return CPFEM_OK;
+}
+
+int cpfem_to_csr(cpfem_ctx* c, void* stream, const double* mat, int64_t* indptr, int32_t* indices, double* data) {
+  if (!c || !mat || !indptr || !indices || !data) return fail(CPFEM_EBADARG, "null argument");
+  std::vector<double> h(c->mat_len());
+  HIPCHK(hipMemcpyAsync(h.data(), mat, h.size() * 8, hipMemcpyDeviceToHost, (hipStream_t)stream));
+  HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+  const int64_t nr = c->n_dofs, nn = c->n_nodes;
+  int64_t nz = 0;
+  indptr[0] = 0;
+  for (int64_t r = 0; r < nr; ++r) {
+    const int64_t i = r / 3;
+    for (int s = 0; s < c->h_nslot[i]; ++s) {
+      const int64_t j = c->h_col[(size_t)s * nn + i];
+      for (int k = 0; k < 3; ++k) {
+        indices[nz] = (int32_t)(3 * j + k);
+        data[nz] = h[(size_t)(3 * s + k) * nr + r];
+        ++nz;
+      }
+    }
+    indptr[r + 1] = nz;
+  }
+  return CPFEM_OK;
+}
+
+int cpfem_bc_x0(cpfem_ctx* c, void* stream, const double* dofs, double* x0) {
+  if (!c || !dofs || !x0) return fail(CPFEM_EBADARG, "null argument");
+  launch_bc_x0(c->n_dofs, c->d_isbc, c->d_ubar, dofs, x0, (hipStream_t)stream);
+  HIPCHK(hipGetLastError());
+  return CPFEM_OK;
+}
+
+}  // extern "C"

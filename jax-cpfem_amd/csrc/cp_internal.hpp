@@ -1,0 +1,122 @@
+// Internal types shared by the kernels and the C-ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "cp_point.hpp"
+
+namespace cpfem {
+
+enum { MP_XM = 0, MP_GSS_A, MP_H, MP_TSAT, MP_R, MP_C11, MP_C12, MP_C44, MP_COUNT };
+
+// Mesh geometry on the device
+struct Geom {
+  const int32_t* cells;  // (n_cells, 8)
+  const double* sg;      // uniform ? [24][8] : [24][n_qp]  (dN_a/dX_d, comp = 3a+d)
+  const double* jxw;     // uniform ? [8] : [n_qp]
+  int64_t n_cells;
+  int64_t n_qp;
+  int uniform;
+};
+
+// Per-point history, SoA with leading dimension ld (= n_qp)
+struct StateView {
+  const double* Fpi;
+  const double* g;
+  const double* slip;
+  const double* rot;
+  int64_t ld;
+};
+struct StateOut {
+  double* Fpi;
+  double* g;
+  double* slip;
+};
+
+// Per-point material parameters; value of point q is ptr[k][q * stride[k]]
+struct MatView {
+  const double* ptr[MP_COUNT];
+  int64_t stride[MP_COUNT];
+  double ao;
+  double dt;
+};
+
+struct SolveCfg {
+  double tol;
+  int max_sub_step;
+  int max_iter;
+};
+
+// Block-ELL matrix pattern (3x3 node blocks, values [(3 s + k)][3 i + c])
+struct BellView {
+  const int32_t* col;    // [n_slots][n_nodes]
+  const int32_t* nslot;  // [n_nodes]
+  int64_t n_nodes;
+  int n_slots;
+};
+
+template <int N>
+void launch_element_qp(const Geom& G, const double* u, const StateView& sv, const MatView& mv, const SlipTable<N>& tab,
+                       const SolveCfg& cfg, double* wP, double* wA, int32_t* iters, int32_t* status, hipStream_t st);
+template <int N>
+void launch_local(int64_t n_qp, const double* Hd, const Geom& G, const double* u, const StateView& sv, const MatView& mv,
+                  const SlipTable<N>& tab, const SolveCfg& cfg, int self_block, double* P, double* A, double* sig,
+                  const StateOut& so, int32_t* iters, int32_t* status, hipStream_t st);
+void launch_element_K(const Geom& G, const double* wP, const double* wA, double* KE, double* RE, hipStream_t st);
+void launch_gather_matrix(const BellView& M, const double* KE, const int32_t* cptr, const int32_t* clist, const uint8_t* is_bc,
+                          double* vals, hipStream_t st);
+void launch_gather_residual(int64_t n_nodes, const double* RE, const int32_t* nptr, const int32_t* nlist, const uint8_t* is_bc,
+                            const double* ubar, const double* u, double* res, hipStream_t st);
+void launch_cell_average(const Geom& G, const double* sig, double* out, hipStream_t st);
+void launch_bc_x0(int64_t n, const uint8_t* is_bc, const double* ubar, const double* u, double* x0, hipStream_t st);
+
+// ---------------------------------------------------------------- Krylov
+// Scalars of one BiCGSTAB run, resident in device memory.
+struct KrylovScalars {
+  double rho, alpha, omega;  // values of the previous iteration
+  double rho_next;           // <rhat, r> for the coming iteration
+  double alpha_new, omega_new;
+  double rs;                 // <r, r>
+  double bs;                 // <b, b>
+  double atol2;
+  double ss;
+  double tol, atol;
+  int32_t k;          // iteration counter (negative: breakdown code)
+  int32_t done;       // read by the host together with k
+  int32_t maxiter;
+  int32_t exit_early;
+  uint32_t counter;          // arrival counter of the finishing block
+  int32_t pad;
+};
+
+struct KrylovWork {
+  int64_t n;
+  int nb;            // blocks of the vector / SpMV kernels
+  double* partial;   // [4][nb]
+  KrylovScalars* sc;
+};
+
+template <typename T>
+struct KVec {
+  T *x, *r, *rhat, *p, *q, *phat, *s, *shat, *t;
+  const T* invd;
+};
+
+template <typename T>
+void krylov_init(const BellView& M, const T* vals, const T* b, KVec<T> v, KrylovWork w, double tol, double atol, int maxiter,
+                 hipStream_t st);
+template <typename T>
+void krylov_iteration(const BellView& M, const T* vals, KVec<T> v, KrylovWork w, hipStream_t st);
+template <typename T>
+void launch_spmv(const BellView& M, const T* vals, const T* x, T* y, hipStream_t st);
+template <typename T>
+void launch_inv_diag(const BellView& M, const T* vals, const int32_t* dslot, T* invd, int precond, hipStream_t st);
+
+void launch_residual_norm(int64_t n, const double* a, const double* b, double* partial, int nb, double* out, uint32_t* counter,
+                          hipStream_t st);
+void launch_axpby(int64_t n, double alpha, const double* x, double beta, double* y, hipStream_t st);
+void launch_cast_f32(int64_t n, const double* x, float* y, hipStream_t st);
+void launch_f32_add(int64_t n, const float* d, double* x, hipStream_t st);
+void launch_residual64(const BellView& M, const double* vals, const double* x, const double* b, double* r, hipStream_t st);
+
+}  // namespace cpfem

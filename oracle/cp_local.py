@@ -298,6 +298,21 @@ def jac_H(ctx, S, aux):
     return dr.reshape(ctx.Q, 9, 9)
 
 
+def _solve(J, b):
+    """Batched dense solve; a singular point yields NaN as jnp.linalg.solve does instead of
+    raising (the local Newton of the reference has no error path)."""
+    try:
+        return np.linalg.solve(J, b)
+    except np.linalg.LinAlgError:
+        out = np.full(np.broadcast_shapes(J.shape[:-2] + (J.shape[-1],), b.shape[:-1]) + b.shape[-1:], np.nan)
+        for i in range(J.shape[0]):
+            try:
+                out[i] = np.linalg.solve(J[i], b[i])
+            except np.linalg.LinAlgError:
+                pass
+        return out
+
+
 # --------------------------------------------------------------------------- a8
 def local_newton(ctx, tol=1e-8, max_sub_step=5, max_iter=200):
     """Local Newton from S = 0 with the "cut-half" backtracking; models_copper.py:204-249.
@@ -323,7 +338,7 @@ def local_newton(ctx, tol=1e-8, max_sub_step=5, max_iter=200):
         nra = nr[idx]
         _, aux = residual(c, ya.reshape(-1, 3, 3))
         J = jac_S(c, ya.reshape(-1, 3, 3), aux)
-        dy = np.linalg.solve(J, -ra[..., None])[..., 0]
+        dy = _solve(J, -ra[..., None])[..., 0]
         relax = np.ones(len(idx))
         crt = ra.copy()
         ncrt = nra.copy()
@@ -368,7 +383,7 @@ def tangent(ctx, S, aux):
     tau, gamma_inc, dgamma, Lp, Fpi, Fe = aux
     JS = jac_S(ctx, S, aux)
     JH = jac_H(ctx, S, aux)
-    dS = -np.linalg.solve(JS, JH).reshape(Q, 3, 3, 9)  # dS[i,j] per direction
+    dS = -_solve(JS, JH).reshape(Q, 3, 3, 9)  # dS[i,j] per direction
     detFpi = np.linalg.det(Fpi)
     Y = Fpi @ S @ np.swapaxes(Fpi, 1, 2) / detFpi[:, None, None]
     invFpi = np.linalg.inv(Fpi)

