@@ -1,0 +1,239 @@
+#!/usr/bin/env python
+"""Benchmark of the crystal-plasticity hot path on one MI355X.
+
+A step is one load step of the polycrystal workload (global Newton to the reference
+tolerances: per-point Kalidindi update + tangent, hex8 assembly into the block-ELL Jacobian,
+Jacobi-BiCGSTAB solves; then the cell-averaged Cauchy stress and the history update), run
+through the drop-in ``solver(problem, solver_options)`` loop.  Default workload: 32^3 hex8,
+262,144 quadrature points, 100 Voronoi grains, FCC12, 2% tension in 40 steps of 5e-4.
+
+value = quadrature points x timed load steps / wall time (whole job, inputs resident in HBM).
+The roofline object refers to the kernel class with the largest total time in the timed
+region, timed with HIP events on its launch stream; the CPU baseline is the oracle timed on
+a bounded sample of the same state on one host core (see the 'sample' string).
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "jax-cpfem_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBS = 8000.0    # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def spmv_bytes(prob, value_bytes=8, vec_bytes=8):
+    """Algorithmic HBM bytes of one Krylov SpMV launch (k_kq / k_kt): block values,
+    one column index per 3x3 block, slot counts, the multiplied vector, one extra vector read
+    (rhat or s) and the result write."""
+    nslot, _ = prob.ctx.pattern()
+    nnzb = int(nslot.sum())
+    n = prob.ctx.n_dofs
+    return 9 * value_bytes * nnzb + 4 * nnzb + 4 * prob.ctx.n_nodes + 3 * vec_bytes * n, nnzb
+
+
+def cpu_baseline(prob, w, params, sol, per_step, sample_cells=512, krylov_iters=16):
+    """Oracle timed on a bounded sample of the state at the start of the timed region."""
+    import numpy as np
+    import torch
+    from threadpoolctl import threadpool_limits
+    from oracle import cp_local, fe
+
+    mat = w.material
+    om = cp_local.Material(cp_local.schmid_tensors(__import__("cpfem_amd").materials.load_slip_table(mat.slip)),
+                           xm=mat.xm, gss_a=mat.gss_a, h=mat.h, t_sat=mat.t_sat, r=mat.r, C11=mat.C11, C12=mat.C12,
+                           C44=mat.C44, self_block=mat.self_block)
+    S = sample_cells
+    cells = w.mesh.cells[:S]
+    u = sol[0].reshape(-1).cpu().numpy()
+    sg, jxw = prob.fes[0].shape_grads[:S], prob.fes[0].JxW[:S]
+    ug = np.einsum("cai,cqaj->cqij", u.reshape(-1, 3)[cells], sg).reshape(-1, 3, 3)
+    take = lambda t, comps: t[:S].reshape((S * 8,) + comps).cpu().numpy()
+    Ns = prob.num_slip_sys
+    Fp, g, sl, R = take(params[0], (3, 3)), take(params[1], (Ns,)), take(params[2], (Ns,)), take(params[3], (3, 3))
+    dt = float(w.ts[1] - w.ts[0])
+    with threadpool_limits(1):
+        t0 = time.perf_counter()
+        out = cp_local.local_update(ug, Fp, g, sl, R, om, dt, max_sub_step=mat.max_sub_step)
+        A = out["A"].reshape(S, 8, 3, 3, 3, 3)
+        P = out["P"].reshape(S, 8, 3, 3)
+        np.einsum("cqaj,cqijkl,cqbl,cq->caibk", sg, A, sg, jxw, optimize=True)
+        np.einsum("cqij,cqaj,cq->cai", P, sg, jxw)
+        t_asm = (time.perf_counter() - t0) / (S * 8)
+        t0 = time.perf_counter()
+        cp_local.local_update(ug, Fp, g, sl, R, om, dt, max_sub_step=mat.max_sub_step, want_tangent=False)
+        t_post = (time.perf_counter() - t0) / (S * 8)
+        res, mat_vals = prob.assemble(sol[0].reshape(-1))
+        Asp = __import__("cpfem_amd").BlockEllMatrix(prob.ctx, mat_vals).to_scipy()
+        b = -res.cpu().numpy()
+        t0 = time.perf_counter()
+        fe.bicgstab_jax(Asp, b, np.zeros_like(b), M=lambda v: v / Asp.diagonal(), maxiter=krylov_iters,
+                        tol=0.0, atol=0.0)
+        t_iter = (time.perf_counter() - t0) / krylov_iters
+    Q = w.n_qp
+    step_s = (per_step["assemblies"] * Q * t_asm + per_step["krylov_iters"] * t_iter + 2 * Q * t_post)
+    return dict(value=Q / step_s, unit="quad-pt load-step updates/s", cores=1, kind="port",
+                sample=(f"oracle (NumPy/SciPy FP64, 1 thread) on {S * 8} points of the timed state "
+                        f"({t_asm * 1e6:.1f} us/point update+tangent+element, {t_post * 1e6:.1f} us/point post) and "
+                        f"{krylov_iters} BiCGSTAB iterations on the full {Asp.shape[0]}-dof Jacobian "
+                        f"({t_iter * 1e3:.1f} ms/iter), scaled by the GPU run's per-step counts "
+                        f"({per_step['assemblies']:.1f} assemblies, {per_step['krylov_iters']:.0f} Krylov iterations); "
+                        f"est. {step_s:.1f} s per load step"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--mesh", type=int, default=None, help="override the mesh size n (n^3 cells)")
+    ap.add_argument("--profile-sample", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--compare-mixed", action="store_true", help="rerun the timed steps in the other precision")
+    ap.add_argument("--copy-calibration", action="store_true", help="also run a plain copy kernel (counter scale)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import cpfem_amd as C
+    from cpfem_amd import _native as N
+    from cpfem_amd import configs
+
+    torch.cuda.set_device(0)
+    kw = {} if args.mesh is None else {"n": args.mesh}
+    w = configs.CONFIGS[args.config](**kw)
+    log(f"[bench] workload {w.name}: {len(w.mesh.cells)} cells, {w.n_qp} points")
+    prob = configs.build_problem(w)
+    L, h = prob.ctx._lib, prob.ctx.h
+    params = prob.internal_vars
+    sol = [torch.zeros(prob.fes[0].num_total_nodes, 3, dtype=torch.float64, device="cuda")]
+    opts = {"precision": args.precision}
+    W, K = args.warmup, args.steps
+    if W + K > len(w.ts) - 1:
+        raise SystemExit(f"warmup + steps exceeds the {len(w.ts) - 1} load steps of the workload")
+    t0 = time.perf_counter()
+    params, sol, _ = configs.run_steps(prob, w, params, sol, 0, W, opts,
+                                       on_step=lambda i, p, s: log(f"[bench] warmup step {i + 1}: szz={s[2, 2]:.6f}"))
+    log(f"[bench] warmup {W} steps {time.perf_counter() - t0:.2f}s")
+    snap = ([p.clone() for p in params], [sol[0].clone()])
+
+    counts = dict(newton=0, krylov=0, solves=0)
+
+    def on_step(i, p, s):
+        counts["newton"] += p.last_solve["newton"]
+        log(f"[bench] step {i + 1}: szz={s[2, 2]:.6f} newton={p.last_solve['newton']} "
+            f"krylov={[d['iterations'] for d in p.solver_info[-p.last_solve['newton']:]] if p.last_solve['newton'] else []}")
+
+    prob.solver_info.clear()
+    N.check(L.cpfem_profile(h, 1, args.profile_sample))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    params, sol, sig = configs.run_steps(prob, w, params, sol, W, W + K, opts, on_step=on_step)
+    torch.cuda.synchronize()
+    T = time.perf_counter() - t0
+    ms = (ctypes.c_double * 8)()
+    la = (ctypes.c_int64 * 8)()
+    ti = (ctypes.c_int64 * 8)()
+    N.check(L.cpfem_profile_read(h, 8, ms, la, ti))
+    N.check(L.cpfem_profile(h, 0, 1))
+    kry = [d["iterations"] for d in prob.solver_info]
+    assemblies = counts["newton"] + K  # one assembly per Newton iteration plus the initial one
+    per_step = dict(assemblies=assemblies / K, krylov_iters=sum(max(k, 0) for k in kry) / K)
+    classes = N.PROF_CLASSES
+    prof = {c: dict(launches=int(la[k]), timed=int(ti[k]), avg_ms=(ms[k] / ti[k] if ti[k] else None),
+                    est_total_ms=(ms[k] / ti[k] * la[k] if ti[k] else 0.0)) for k, c in enumerate(classes)}
+    dom = max(prof, key=lambda c: prof[c]["est_total_ms"])
+    log(f"[bench] timed {K} steps in {T:.3f}s; profile: " +
+        ", ".join(f"{c}={prof[c]['est_total_ms']:.1f}ms/{prof[c]['launches']}" for c in classes))
+
+    Q = w.n_qp
+    value = Q * K / T
+    vb = 4 if args.precision == "mixed" else 8
+    sb, nnzb = spmv_bytes(prob, value_bytes=vb, vec_bytes=vb)
+    roof = None
+    if dom == "krylov_spmv" and prof[dom]["avg_ms"]:
+        ach = sb / (prof[dom]["avg_ms"] * 1e-3) / 1e9
+        roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4),
+                    traffic=None, kernel="k_kq/k_kt (Krylov SpMV with fused dots)",
+                    bytes_per_launch=sb, avg_launch_us=round(prof[dom]["avg_ms"] * 1e3, 2))
+    elif prof[dom]["avg_ms"]:
+        roof = dict(bound="mfma" if False else "hbm", achieved=None, peak=None, unit=None, frac=None, traffic=None,
+                    kernel=dom, avg_launch_us=round(prof[dom]["avg_ms"] * 1e3, 2))
+
+    detail = dict(
+        kernel_profile=prof, dominant=dom, newton_per_step=counts["newton"] / K, krylov_per_solve=kry,
+        point_evals_per_s_in_update_kernel=(Q * prof["element_qp"]["launches"] / (prof["element_qp"]["est_total_ms"] * 1e-3)
+                                            if prof["element_qp"]["est_total_ms"] else None),
+        sigma_bar_zz=[float(s[2, 2]) for s in sig], nnz_blocks=nnzb, n_dofs=prob.ctx.n_dofs,
+        spmv_bytes_per_launch=sb, load_step_s=T / K)
+
+    if args.compare_mixed:
+        other = "mixed" if args.precision == "fp64" else "fp64"
+        p2, s2 = snap
+        prob.solver_info.clear()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        _, _, sig2 = configs.run_steps(prob, w, p2, s2, W, W + K, {"precision": other})
+        torch.cuda.synchronize()
+        T2 = time.perf_counter() - t1
+        ref = np.array([s[2, 2] for s in (sig if args.precision == "fp64" else sig2)])
+        mix = np.array([s[2, 2] for s in (sig2 if args.precision == "fp64" else sig)])
+        detail["compare"] = dict(other_precision=other, other_load_step_s=T2 / K,
+                                 speedup_mixed_over_fp64=(T / T2 if other == "mixed" else T2 / T),
+                                 max_rel_err_sigma_zz=float(np.max(np.abs(mix - ref) / np.abs(ref))),
+                                 other_krylov=[d["iterations"] for d in prob.solver_info])
+        log(f"[bench] {other}: {T2 / K:.3f}s/step, rel err {detail['compare']['max_rel_err_sigma_zz']:.2e}")
+
+    if args.copy_calibration:
+        n = 1 << 26
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        N.check(L.cpfem_profile(h, 1, 1))
+        for _ in range(5):
+            N.check(L.cpfem_copy(h, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream), ctypes.c_void_p(src.data_ptr()),
+                                 ctypes.c_void_p(dst.data_ptr()), n))
+        N.check(L.cpfem_profile_read(h, 8, ms, la, ti))
+        N.check(L.cpfem_profile(h, 0, 1))
+        cms = ms[7] / ti[7]
+        detail["copy_calibration"] = dict(bytes=16 * n, avg_ms=cms, GBps=16 * n / (cms * 1e-3) / 1e9)
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle sample) ...")
+        cpu = cpu_baseline(prob, w, snap[0], snap[1], per_step)
+        log(f"[bench] cpu {cpu['value']:.1f} {cpu['unit']}")
+
+    line = {
+        "metric": "CP quad-pt updates/s, load-step time, adjoint-grad time (mixed vs FP64)",
+        "value": round(value, 1),
+        "unit": "quad-pt load-step updates/s",
+        "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": round(T / K * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f64+f32-krylov",
+        "data": "synthetic (seeded Voronoi polycrystal, uniform SO(3) orientations)",
+        "config": {"workload": w.name, "n_qp": Q, "load_steps": f"{W + 1}..{W + K} of {len(w.ts) - 1}",
+                   "precision": args.precision, "solver": "row-elimination Newton, Jacobi-BiCGSTAB tol=atol=1e-10"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "detail": detail,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -160,6 +160,16 @@ int cpfem_axpby(cpfem_ctx* ctx, void* stream, double alpha, const double* x, dou
 /* CSR export of the block-ELL matrix (host outputs: indptr n_dofs+1, indices/data nnz). */
 int cpfem_to_csr(cpfem_ctx* ctx, void* stream, const double* mat, int64_t* indptr, int32_t* indices, double* data);
 
+/* Kernel-class timing with HIP events on the launch stream (every `sample`-th launch of a
+   class is timed, all are counted); enable resets the counters.  Classes (index):
+   0 point update + tangent, 1 element stiffness/residual, 2 matrix gather, 3 residual
+   gather, 4 Krylov SpMV kernels, 5 Krylov vector kernels, 6 point stress/history, 7 copy. */
+int cpfem_profile(cpfem_ctx* ctx, int32_t enable, int32_t sample);
+int cpfem_profile_read(cpfem_ctx* ctx, int32_t n, double* ms, int64_t* launches, int64_t* timed);
+
+/* Plain device copy of n doubles (bandwidth / counter calibration). */
+int cpfem_copy(cpfem_ctx* ctx, void* stream, const double* src, double* dst, int64_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -71,7 +71,13 @@ _SIGS = {
     "cpfem_norm": (ctypes.c_int, [vp, vp, vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32)]),
     "cpfem_axpby": (ctypes.c_int, [vp, vp, ctypes.c_double, vp, ctypes.c_double, vp]),
     "cpfem_to_csr": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),
+    "cpfem_profile": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32]),
+    "cpfem_profile_read": (ctypes.c_int, [vp, ctypes.c_int32, vp, vp, vp]),
+    "cpfem_copy": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_int64]),
 }
+
+PROF_CLASSES = ("element_qp", "element_K", "gather_matrix", "gather_residual", "krylov_spmv", "krylov_vector",
+                "point_local", "copy")
 
 EXPORTED = tuple(_SIGS)
 _lib = None

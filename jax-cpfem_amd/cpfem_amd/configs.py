@@ -1,0 +1,165 @@
+"""The five benchmark workloads as concrete synthetic inputs (units MPa, mm, s).
+
+Each builder returns a Workload: mesh, material, orientations (scalar-first quaternions) and
+per-cell orientation indices, the symmetry boundary conditions of
+polycrystal_DPsteel_inhomo.py:180-182 (x=0: u_x=0, y=0: u_y=0, z=0: u_z=0, z=L: u_z=eps L),
+and the displacement / time histories (strain rate 1e-3 /s, so dt = d_eps / 1e-3).
+Randomness: numpy default_rng(seed) (PCG64); Voronoi seeds are drawn first, then the grain
+quaternions (Shoemake), then any phase permutation.
+"""
+import dataclasses
+
+import numpy as np
+
+from . import materials as M
+from .generate_mesh import Mesh, box_mesh, voronoi_grains
+from .rotations import uniform_quaternions
+
+
+@dataclasses.dataclass
+class Workload:
+    name: str
+    mesh: Mesh
+    material: M.Material
+    quat: np.ndarray
+    cell_ori_inds: np.ndarray
+    disps: np.ndarray
+    ts: np.ndarray
+    L: float = 1.0
+    phase_inds: np.ndarray = None
+    phase_materials: tuple = None
+    precision: str = "fp64"
+
+    @property
+    def n_qp(self):
+        return 8 * len(self.mesh.cells)
+
+    def dirichlet_bc_info(self, disp):
+        L = self.L
+        left = lambda p: np.isclose(p[0], 0.0, atol=1e-5)
+        front = lambda p: np.isclose(p[1], 0.0, atol=1e-5)
+        bottom = lambda p: np.isclose(p[2], 0.0, atol=1e-5)
+        top = lambda p: np.isclose(p[2], L, atol=1e-5)
+        zero = lambda p: 0.0
+        return [[left, front, bottom, top], [0, 1, 2, 2], [zero, zero, zero, (lambda d: (lambda p: d))(disp)]]
+
+
+def _mesh(n, L=1.0):
+    m = box_mesh(n, n, n, L, L, L)
+    return Mesh(m.points, m.cells_dict["hexahedron"])
+
+
+def _grains(mesh, n_grains, rng, L=1.0):
+    cent = mesh.points[mesh.cells].mean(axis=1)
+    ids, _ = voronoi_grains(cent, n_grains, (L, L, L), rng)
+    quat = uniform_quaternions(n_grains, rng)
+    return quat, ids
+
+
+def _history(total_strain, steps, L=1.0, rate=1e-3):
+    disps = np.linspace(0.0, total_strain * L, steps + 1)
+    ts = np.linspace(0.0, total_strain / rate, steps + 1)
+    return disps, ts
+
+
+def config1(seed=0):
+    """Single-crystal FCC copper, 8^3 hex8 (4,096 points), 1% tension in 10 steps."""
+    rng = np.random.default_rng(seed)
+    mesh = _mesh(8)
+    quat = uniform_quaternions(1, rng)
+    mat = M.Material(slip="fcc12", g0=16.0, h=180.0, t_sat=148.0, gss_a=2.25, xm=0.02, r=1.4,
+                     C11=168.4e3, C12=121.4e3, C44=75.4e3)
+    d, t = _history(0.01, 10)
+    return Workload("C1 Cu single crystal 8^3 FP64", mesh, mat, quat, np.zeros(len(mesh.cells), np.int64), d, t)
+
+
+def config2(seed=0, n=32):
+    """Polycrystal FCC 304-steel-like, 32^3 hex8 (262,144 points), 100 Voronoi grains,
+    2% tension in 40 steps."""
+    rng = np.random.default_rng(seed)
+    mesh = _mesh(n)
+    quat, ids = _grains(mesh, 100, rng)
+    mat = M.Material(slip="fcc12", g0=88.0, h=500.0, t_sat=300.0, gss_a=2.0, xm=0.02, r=1.4,
+                     C11=204.6e3, C12=137.7e3, C44=126.2e3)
+    d, t = _history(0.02, 40)
+    return Workload(f"C2 FCC polycrystal {n}^3, 100 grains, FP64", mesh, mat, quat, ids, d, t)
+
+
+def config3(seed=0, n=40):
+    """Polycrystal BCC tantalum-like, 40^3 (512,000 points), 64 grains, BCC24, 3% in 30 steps;
+    mixed precision against plain FP64."""
+    rng = np.random.default_rng(seed)
+    mesh = _mesh(n)
+    quat, ids = _grains(mesh, 64, rng)
+    mat = M.Material(slip="bcc24", g0=150.0, h=800.0, t_sat=400.0, gss_a=2.0, xm=0.02, r=1.4,
+                     C11=266.0e3, C12=161.0e3, C44=82.5e3)
+    d, t = _history(0.03, 30)
+    return Workload(f"C3 BCC polycrystal {n}^3, 64 grains", mesh, mat, quat, ids, d, t, precision="mixed")
+
+
+def config4(seed=0, n=24):
+    """Adjoint calibration workload: 24^3, 64 grains, FCC with 304-like stiffness; hidden
+    parameters (90, 450, 280, 2.2, 0.02), 5% in 50 steps."""
+    rng = np.random.default_rng(seed)
+    mesh = _mesh(n)
+    quat, ids = _grains(mesh, 64, rng)
+    mat = M.Material(slip="fcc12", g0=90.0, h=450.0, t_sat=280.0, gss_a=2.2, xm=0.02, r=1.4,
+                     C11=204.6e3, C12=137.7e3, C44=126.2e3, layout="9")
+    d, t = _history(0.05, 50)
+    return Workload(f"C4 calibration {n}^3, 64 grains", mesh, mat, quat, ids, d, t)
+
+
+def config5(seed=0, n=64):
+    """Dual-phase BCC24 orientation-design workload: 64^3 (2,097,152 points), 300 grains,
+    20% of grains hard (g0 x4, h0 x2), 2% in 20 steps."""
+    rng = np.random.default_rng(seed)
+    mesh = _mesh(n)
+    quat, ids = _grains(mesh, 300, rng)
+    soft = M.Material(slip="bcc24", g0=100.0, h=400.0, t_sat=250.0, gss_a=2.0, xm=0.02, r=1.4,
+                      C11=231.0e3, C12=135.0e3, C44=116.0e3, layout="10")
+    hard = soft.replace(g0=400.0, h=800.0)
+    hard_grains = rng.permutation(300)[:60]
+    grain_phase = np.zeros(300, dtype=np.int64)
+    grain_phase[hard_grains] = 1
+    d, t = _history(0.02, 20)
+    return Workload(f"C5 dual-phase BCC {n}^3, 300 grains", mesh, soft, quat, ids, d, t,
+                    phase_inds=grain_phase[ids], phase_materials=(soft, hard), precision="mixed")
+
+
+CONFIGS = {1: config1, 2: config2, 3: config3, 4: config4, 5: config5}
+
+
+def build_problem(w):
+    """CrystalPlasticity for a workload (BCs at the first displacement)."""
+    from .models import CrystalPlasticity
+    info = w.dirichlet_bc_info(w.disps[0])
+    extra = (w.phase_inds, w.phase_materials) if w.phase_inds is not None else ()
+    prob = CrystalPlasticity(w.mesh, vec=3, dim=3, ele_type="HEX8", dirichlet_bc_info=info,
+                             additional_info=(w.quat, w.cell_ori_inds) + extra, material=w.material)
+    prob._bc_info = info
+    return prob
+
+
+def run_steps(prob, w, params, sol, i0, i1, solver_options=None, on_step=None):
+    """The reference driver loop (singlecrystal_copper.py:179-233) over steps [i0, i1).
+    Returns (params, sol, sigma_bar list (3x3 numpy per step))."""
+    import torch
+    from .solver import solver
+    opts = dict(solver_options or {})
+    out = []
+    info = prob._bc_info
+    for i in range(i0, i1):
+        prob.dt = w.ts[i + 1] - w.ts[i]
+        info[-1][-1] = (lambda d: (lambda p: d))(w.disps[i + 1])
+        prob.fes[0].update_Dirichlet_boundary_conditions(info)
+        prob.set_params(params)
+        o = {"jax_solver": {}, "initial_guess": sol}
+        o.update(opts)
+        sol = solver(prob, o)
+        sig = prob.compute_avg_stress(sol[0], params)
+        out.append(sig.mean(dim=0).cpu().numpy())
+        params = prob.update_int_vars_gp(sol[0], params)
+        if on_step is not None:
+            on_step(i, prob, out[-1])
+    torch.cuda.synchronize()
+    return params, sol, out

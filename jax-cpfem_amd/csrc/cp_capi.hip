@@ -11,6 +11,51 @@
 #include "../../include/cpfem.h"
 #include "cp_internal.hpp"
 
+namespace cpfem {
+
+static hipEvent_t prof_event(Prof* p) {
+  if (!p->pool.empty()) {
+    hipEvent_t e = p->pool.back();
+    p->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+int prof_begin(Prof* p, int cls, hipStream_t st) {
+  if (!p || !p->on) return -1;
+  const int64_t k = p->launches[cls]++;
+  if (k % p->sample) return -1;
+  if (p->pend.size() >= 8192) prof_flush(p);
+  Prof::Pending e{cls, prof_event(p), prof_event(p)};
+  (void)hipEventRecord(e.a, st);
+  p->pend.push_back(e);
+  return (int)p->pend.size() - 1;
+}
+
+void prof_end(Prof* p, int slot, hipStream_t st) {
+  if (!p || slot < 0) return;
+  (void)hipEventRecord(p->pend[slot].b, st);
+}
+
+int prof_flush(Prof* p) {
+  for (auto& e : p->pend) {
+    float ms = 0.f;
+    if (hipEventSynchronize(e.b) == hipSuccess && hipEventElapsedTime(&ms, e.a, e.b) == hipSuccess) {
+      p->ms[e.cls] += ms;
+      p->timed[e.cls] += 1;
+    }
+    p->pool.push_back(e.a);
+    p->pool.push_back(e.b);
+  }
+  p->pend.clear();
+  return 0;
+}
+
+}  // namespace cpfem
+
 using namespace cpfem;
 
 static thread_local std::string g_err;
@@ -67,16 +112,16 @@ struct cpfem_ctx {
   uint32_t* d_counter = nullptr;
   // Krylov
   KrylovScalars* d_ksc = nullptr;
-  double* kv64[10] = {};
-  float* kv32[10] = {};
+  double* d_kpartial = nullptr;  // [PS_COUNT][krylov blocks]
+  double* kv64[10] = {};  // r rhat p q phat s shat t invd d
   float* d_mat32 = nullptr;
   double *d_r64 = nullptr, *d_tmp64 = nullptr;
-  float* d_b32 = nullptr;
   // pinned host staging
   KrylovScalars* h_ksc = nullptr;
   double* h_dbl = nullptr;
   int32_t* h_int = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
+  Prof prof;
 
   Geom geom() const {
     Geom G;
@@ -317,17 +362,13 @@ static int ensure_krylov(cpfem_ctx* c, bool mixed) {
   int rc;
   const int64_t n = c->n_dofs;
   if (!c->d_ksc) {
-    if ((rc = dalloc(&c->d_ksc, 1))) return rc;
+    if ((rc = dalloc(&c->d_ksc, 1)) || (rc = dalloc(&c->d_kpartial, (size_t)8 * krylov_blocks(n)))) return rc;
     for (int i = 0; i < 10; ++i)
       if ((rc = dalloc(&c->kv64[i], n))) return rc;
     if ((rc = dalloc(&c->d_r64, n)) || (rc = dalloc(&c->d_tmp64, n))) return rc;
     HIPCHK(hipMemset(c->d_ksc, 0, sizeof(KrylovScalars)));
   }
-  if (mixed && !c->d_mat32) {
-    for (int i = 0; i < 10; ++i)
-      if ((rc = dalloc(&c->kv32[i], n))) return rc;
-    if ((rc = dalloc(&c->d_mat32, c->mat_len())) || (rc = dalloc(&c->d_b32, n))) return rc;
-  }
+  if (mixed && !c->d_mat32 && (rc = dalloc(&c->d_mat32, c->mat_len()))) return rc;
   return CPFEM_OK;
 }
 
@@ -341,9 +382,8 @@ static int dot_host(cpfem_ctx* c, hipStream_t st, const double* a, const double*
   return CPFEM_OK;
 }
 
-template <typename T>
-static KVec<T> kvec(T** v, T* x) {
-  KVec<T> k;
+static KVec kvec(double** v, double* x) {
+  KVec k;
   k.x = x;
   k.r = v[0];
   k.rhat = v[1];
@@ -357,12 +397,14 @@ static KVec<T> kvec(T** v, T* x) {
   return k;
 }
 
-// One BiCGSTAB run (JAX semantics) on the ctx's scalar block; returns iteration counter.
-template <typename T>
-static int run_bicgstab(cpfem_ctx* c, hipStream_t st, const T* vals, const T* b, T* x, T** vecs, double tol, double atol,
-                        int maxiter, int precond, int32_t* k_out) {
+// One BiCGSTAB run (JAX semantics) with the Jacobi vector already in kv64[8]; returns the
+// iteration counter (negative: breakdown code).
+// With vals64 != nullptr (mixed form) the recurrence residual is replaced by the true FP64
+// residual after every batch (reliable updates).
+template <typename TM>
+static int run_bicgstab(cpfem_ctx* c, hipStream_t st, const TM* vals, const double* b, double* x, double tol, double atol,
+                        int maxiter, int32_t* k_out, const double* vals64 = nullptr) {
   const BellView M = c->bell();
-  launch_inv_diag<T>(M, vals, c->d_dslot, vecs[8], precond, st);
   KrylovScalars h{};
   h.tol = tol;
   h.atol = atol;
@@ -371,18 +413,25 @@ static int run_bicgstab(cpfem_ctx* c, hipStream_t st, const T* vals, const T* b,
   HIPCHK(hipMemcpyAsync(c->d_ksc, c->h_ksc, sizeof(KrylovScalars), hipMemcpyHostToDevice, st));
   KrylovWork w;
   w.n = c->n_dofs;
-  w.nb = c->nb;
-  w.partial = c->d_partial;
+  w.nb = krylov_blocks(c->n_dofs);
+  w.partial = c->d_kpartial;
   w.sc = c->d_ksc;
-  KVec<T> v = kvec<T>(vecs, x);
-  krylov_init<T>(M, vals, b, v, w, tol, atol, maxiter, st);
+  KVec v = kvec(c->kv64, x);
+  int sl = prof_begin(&c->prof, PC_SPMV, st);
+  krylov_init<TM>(M, vals, b, v, w, st);
+  prof_end(&c->prof, sl, st);
   HIPCHK(hipGetLastError());
   // Batches of iterations; the done flag of batch j is read while batch j+1 runs.
   const int CH = 16;
   int issued = 0, batch = 0;
   bool done = false;
   while (!done) {
-    for (int i = 0; i < CH; ++i) krylov_iteration<T>(M, vals, v, w, st);
+    for (int i = 0; i < CH; ++i) krylov_iteration<TM>(M, vals, v, w, i & 1, st, &c->prof);
+    if (vals64) {
+      const int sr = prof_begin(&c->prof, PC_SPMV, st);
+      krylov_replace(M, vals64, b, v, w, st);
+      prof_end(&c->prof, sr, st);
+    }
     issued += CH;
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(&c->h_int[2 * (batch & 1)], &c->d_ksc->k, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -478,13 +527,16 @@ void cpfem_destroy(cpfem_ctx* c) {
   dfree(c->d_wP); dfree(c->d_wA); dfree(c->d_KE); dfree(c->d_RE); dfree(c->d_sig); dfree(c->d_status);
   dfree(c->d_partial); dfree(c->d_scalar); dfree(c->d_counter);
   dfree(c->d_ksc);
-  for (int i = 0; i < 10; ++i) { dfree(c->kv64[i]); dfree(c->kv32[i]); }
-  dfree(c->d_mat32); dfree(c->d_r64); dfree(c->d_tmp64); dfree(c->d_b32);
+  dfree(c->d_kpartial);
+  for (int i = 0; i < 10; ++i) dfree(c->kv64[i]);
+  dfree(c->d_mat32); dfree(c->d_r64); dfree(c->d_tmp64);
   if (c->h_ksc) (void)hipHostFree(c->h_ksc);
   if (c->h_dbl) (void)hipHostFree(c->h_dbl);
   if (c->h_int) (void)hipHostFree(c->h_int);
   for (int i = 0; i < 2; ++i)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+  prof_flush(&c->prof);
+  for (auto e : c->prof.pool) (void)hipEventDestroy(e);
   delete c;
 }
 
@@ -553,10 +605,12 @@ int cpfem_local(cpfem_ctx* c, void* stream, int64_t n_qp, const double* u_grad, 
   const StateView sv = state_view(state, n_qp);
   const MatView mv = mat_view(params);
   const SolveCfg cfg = solve_cfg(params);
+  const int sl = prof_begin(&c->prof, PC_LOCAL, st);
   if (c->n_slip == 12)
     launch_local<12>(n_qp, u_grad, G, dofs, sv, mv, c->tab12, cfg, params->self_block, P, A, sigma, so, iters, c->d_status, st);
   else
     launch_local<24>(n_qp, u_grad, G, dofs, sv, mv, c->tab24, cfg, params->self_block, P, A, sigma, so, iters, c->d_status, st);
+  prof_end(&c->prof, sl, st);
   HIPCHK(hipGetLastError());
   return CPFEM_OK;
 }
@@ -573,14 +627,25 @@ int cpfem_assemble(cpfem_ctx* c, void* stream, const double* dofs, const CpfemSt
   const MatView mv = mat_view(params);
   const SolveCfg cfg = solve_cfg(params);
   double* wA = mat ? c->d_wA : nullptr;
+  Prof* pf = &c->prof;
+  int sl = prof_begin(pf, PC_ELEMENT_QP, st);
   if (c->n_slip == 12)
     launch_element_qp<12>(G, dofs, sv, mv, c->tab12, cfg, c->d_wP, wA, iters, c->d_status, st);
   else
     launch_element_qp<24>(G, dofs, sv, mv, c->tab24, cfg, c->d_wP, wA, iters, c->d_status, st);
+  prof_end(pf, sl, st);
+  sl = prof_begin(pf, PC_ELEMENT_K, st);
   launch_element_K(G, c->d_wP, wA, c->d_KE, c->d_RE, st);
+  prof_end(pf, sl, st);
   const uint8_t* isbc = (flags & CPFEM_APPLY_BC) ? c->d_isbc : c->d_nobc;
+  sl = prof_begin(pf, PC_GATHER_RES, st);
   launch_gather_residual(c->n_nodes, c->d_RE, c->d_nptr, c->d_nlist, isbc, c->d_ubar, dofs, res, st);
-  if (mat) launch_gather_matrix(c->bell(), c->d_KE, c->d_cptr, c->d_clist, isbc, mat, st);
+  prof_end(pf, sl, st);
+  if (mat) {
+    sl = prof_begin(pf, PC_GATHER_MAT, st);
+    launch_gather_matrix(c->bell(), c->d_KE, c->d_cptr, c->d_clist, isbc, mat, st);
+    prof_end(pf, sl, st);
+  }
   HIPCHK(hipGetLastError());
   return CPFEM_OK;
 }
@@ -606,7 +671,7 @@ int cpfem_update_state(cpfem_ctx* c, void* stream, const double* dofs, const Cpf
 
 int cpfem_spmv(cpfem_ctx* c, void* stream, const double* mat, const double* x, double* y) {
   if (!c || !mat || !x || !y) return fail(CPFEM_EBADARG, "null argument");
-  launch_spmv<double>(c->bell(), mat, x, y, (hipStream_t)stream);
+  launch_spmv(c->bell(), mat, x, y, (hipStream_t)stream);
   HIPCHK(hipGetLastError());
   return CPFEM_OK;
 }
@@ -646,32 +711,30 @@ int cpfem_solve(cpfem_ctx* c, void* stream, const double* mat, const double* b, 
   double bb;
   if ((rc = dot_host(c, st, b, b, &bb))) return rc;
   inf.b_norm = std::sqrt(bb);
+  launch_inv_diag(c->bell(), mat, c->d_dslot, c->kv64[8], precond, st);
   if (precision == CPFEM_FP64) {
     int32_t k = 0;
-    if ((rc = run_bicgstab<double>(c, st, mat, b, x, c->kv64, tol, atol, maxiter, precond, &k))) return rc;
+    if ((rc = run_bicgstab<double>(c, st, mat, b, x, tol, atol, maxiter, &k))) return rc;
     inf.iterations = k;
     inf.outer = 1;
   } else {
-    // FP32 inner BiCGSTAB on a rounded copy of A, FP64 iterative refinement on the true
-    // residual until |b - A x| <= max(tol |b|, atol).
+    // One BiCGSTAB whose SpMVs read an FP32 copy of the matrix (vectors and scalars FP64),
+    // with the recurrence residual replaced by the FP64 residual b - A x every batch; a
+    // restart from the current x follows only if the true residual is still above
+    // max(tol |b|, atol) when the iteration stops.
     const BellView M = c->bell();
     launch_cast_f32(c->mat_len(), mat, c->d_mat32, st);
     const double thr2 = std::max(tol * tol * bb, atol * atol);
     int total = 0;
     int outer = 0;
-    for (; outer < 30; ++outer) {
+    for (; outer < 8; ++outer) {
+      int32_t k = 0;
+      if ((rc = run_bicgstab<float>(c, st, c->d_mat32, b, x, tol, atol, maxiter - total, &k, mat))) return rc;
+      total += (k > 0 ? k : 0);
       launch_residual64(M, mat, x, b, c->d_r64, st);
       double rr;
       if ((rc = dot_host(c, st, c->d_r64, c->d_r64, &rr))) return rc;
-      if (!(rr > thr2) || total >= maxiter) break;
-      launch_cast_f32(n, c->d_r64, c->d_b32, st);
-      HIPCHK(hipMemsetAsync(c->kv32[9], 0, n * sizeof(float), st));
-      int32_t k = 0;
-      if ((rc = run_bicgstab<float>(c, st, c->d_mat32, c->d_b32, c->kv32[9], c->kv32, 1e-5, 0.0, maxiter - total, precond, &k)))
-        return rc;
-      total += (k > 0 ? k : 0);
-      launch_f32_add(n, c->kv32[9], x, st);
-      if (k <= 0) { ++outer; launch_residual64(M, mat, x, b, c->d_r64, st); break; }
+      if (!(rr > thr2) || total >= maxiter || k <= 0) { ++outer; break; }
     }
     inf.iterations = total;
     inf.outer = outer;
@@ -711,6 +774,40 @@ int cpfem_to_csr(cpfem_ctx* c, void* stream, const double* mat, int64_t* indptr,
     }
     indptr[r + 1] = nz;
   }
+  return CPFEM_OK;
+}
+
+int cpfem_profile(cpfem_ctx* c, int32_t enable, int32_t sample) {
+  if (!c || sample < 1) return fail(CPFEM_EBADARG, "bad profile arguments");
+  prof_flush(&c->prof);
+  for (int k = 0; k < PC_COUNT; ++k) {
+    c->prof.launches[k] = 0;
+    c->prof.timed[k] = 0;
+    c->prof.ms[k] = 0.0;
+  }
+  c->prof.on = enable != 0;
+  c->prof.sample = sample;
+  return CPFEM_OK;
+}
+
+int cpfem_profile_read(cpfem_ctx* c, int32_t n, double* ms, int64_t* launches, int64_t* timed) {
+  if (!c || !ms || !launches || !timed || n < 1) return fail(CPFEM_EBADARG, "null argument");
+  prof_flush(&c->prof);
+  for (int k = 0; k < n && k < PC_COUNT; ++k) {
+    ms[k] = c->prof.ms[k];
+    launches[k] = c->prof.launches[k];
+    timed[k] = c->prof.timed[k];
+  }
+  return CPFEM_OK;
+}
+
+int cpfem_copy(cpfem_ctx* c, void* stream, const double* src, double* dst, int64_t n) {
+  if (!c || !src || !dst || n < 0) return fail(CPFEM_EBADARG, "bad copy arguments");
+  hipStream_t st = (hipStream_t)stream;
+  const int sl = prof_begin(&c->prof, PC_COPY, st);
+  launch_copy(n, src, dst, st);
+  prof_end(&c->prof, sl, st);
+  HIPCHK(hipGetLastError());
   return CPFEM_OK;
 }
 

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "cp_point.hpp"
 
 namespace cpfem {
@@ -70,22 +72,50 @@ void launch_gather_residual(int64_t n_nodes, const double* RE, const int32_t* np
 void launch_cell_average(const Geom& G, const double* sig, double* out, hipStream_t st);
 void launch_bc_x0(int64_t n, const uint8_t* is_bc, const double* ubar, const double* u, double* x0, hipStream_t st);
 
+// ---------------------------------------------------------------- profiling
+// Optional per-kernel-class timing with HIP events recorded on the launch stream around
+// every `sample`-th launch of a class; the rest are only counted.
+enum ProfClass {
+  PC_ELEMENT_QP = 0,  // constitutive update + tangent (k_element_qp)
+  PC_ELEMENT_K,       // element stiffness / residual (k_element_K, k_element_R)
+  PC_GATHER_MAT,      // block-ELL gather (k_gather_matrix)
+  PC_GATHER_RES,      // residual gather (k_gather_residual)
+  PC_SPMV,            // Krylov kernels containing the SpMV (k_kq, k_kt, k_kinit)
+  PC_KVEC,            // Krylov vector kernels (k_kp, k_ks, k_kxr)
+  PC_LOCAL,           // point kernel for stress / history (k_local)
+  PC_COPY,            // plain copy (bandwidth calibration)
+  PC_COUNT
+};
+
+struct Prof {
+  bool on = false;
+  int sample = 1;
+  int64_t launches[PC_COUNT] = {};
+  int64_t timed[PC_COUNT] = {};
+  double ms[PC_COUNT] = {};
+  struct Pending { int cls; hipEvent_t a, b; };
+  std::vector<Pending> pend;   // recorded, not yet read
+  std::vector<hipEvent_t> pool;  // free events
+};
+
+// Returns the event pair slot to record around the launch, or -1 (not sampled).
+int prof_begin(Prof* p, int cls, hipStream_t st);
+void prof_end(Prof* p, int slot, hipStream_t st);
+int prof_flush(Prof* p);  // synchronises pending events into ms[]
+
 // ---------------------------------------------------------------- Krylov
-// Scalars of one BiCGSTAB run, resident in device memory.
+// Scalars of one BiCGSTAB run, resident in device memory.  Per-iteration values live in
+// two parity slots (iteration j writes [j & 1], reads [(j + 1) & 1]).
 struct KrylovScalars {
-  double rho, alpha, omega;  // values of the previous iteration
-  double rho_next;           // <rhat, r> for the coming iteration
-  double alpha_new, omega_new;
-  double rs;                 // <r, r>
-  double bs;                 // <b, b>
+  double rho2[2], alpha2[2], omega2[2];
+  double rs;     // <r, r> at the last convergence test
+  double bs;     // <b, b>
   double atol2;
-  double ss;
   double tol, atol;
-  int32_t k;          // iteration counter (negative: breakdown code)
-  int32_t done;       // read by the host together with k
+  int32_t k2[2];
+  int32_t k;     // iteration counter at the last test (negative: breakdown code)
+  int32_t done;  // read by the host together with k
   int32_t maxiter;
-  int32_t exit_early;
-  uint32_t counter;          // arrival counter of the finishing block
   int32_t pad;
 };
 
@@ -96,27 +126,25 @@ struct KrylovWork {
   KrylovScalars* sc;
 };
 
-template <typename T>
 struct KVec {
-  T *x, *r, *rhat, *p, *q, *phat, *s, *shat, *t;
-  const T* invd;
+  double *x, *r, *rhat, *p, *q, *phat, *s, *shat, *t;
+  const double* invd;
 };
 
-template <typename T>
-void krylov_init(const BellView& M, const T* vals, const T* b, KVec<T> v, KrylovWork w, double tol, double atol, int maxiter,
-                 hipStream_t st);
-template <typename T>
-void krylov_iteration(const BellView& M, const T* vals, KVec<T> v, KrylovWork w, hipStream_t st);
-template <typename T>
-void launch_spmv(const BellView& M, const T* vals, const T* x, T* y, hipStream_t st);
-template <typename T>
-void launch_inv_diag(const BellView& M, const T* vals, const int32_t* dslot, T* invd, int precond, hipStream_t st);
+template <typename TM>
+void krylov_init(const BellView& M, const TM* vals, const double* b, KVec v, KrylovWork w, hipStream_t st);
+template <typename TM>
+void krylov_iteration(const BellView& M, const TM* vals, KVec v, KrylovWork w, int par, hipStream_t st, Prof* prof);
+int krylov_blocks(int64_t n);
+void krylov_replace(const BellView& M, const double* vals64, const double* b, KVec v, KrylovWork w, hipStream_t st);
+void launch_spmv(const BellView& M, const double* vals, const double* x, double* y, hipStream_t st);
+void launch_inv_diag(const BellView& M, const double* vals, const int32_t* dslot, double* invd, int precond, hipStream_t st);
 
 void launch_residual_norm(int64_t n, const double* a, const double* b, double* partial, int nb, double* out, uint32_t* counter,
                           hipStream_t st);
 void launch_axpby(int64_t n, double alpha, const double* x, double beta, double* y, hipStream_t st);
 void launch_cast_f32(int64_t n, const double* x, float* y, hipStream_t st);
-void launch_f32_add(int64_t n, const float* d, double* x, hipStream_t st);
 void launch_residual64(const BellView& M, const double* vals, const double* x, const double* b, double* r, hipStream_t st);
+void launch_copy(int64_t n, const double* x, double* y, hipStream_t st);
 
 }  // namespace cpfem

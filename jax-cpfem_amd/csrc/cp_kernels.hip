@@ -149,7 +149,8 @@ __global__ void __launch_bounds__(256) k_element_K(Geom G, const double* __restr
 }
 
 // ---------------------------------------------------------------------------------
-// Block-ELL gather: thread per (slot s, scalar row r = 3 i + c).  Sums the element blocks
+// Block-ELL gather: thread per (slot s, scalar row r = 3 i + c), padding slots included.
+// Sums the element blocks
 // that touch (node i, node col[s][i]) in ascending cell order (the order in which scipy's
 // COO->CSR sum_duplicates adds them, solver.py:281-282), then applies the row elimination
 // of get_A (solver.py:290-293): a Dirichlet row becomes zero with a unit diagonal.
@@ -163,9 +164,10 @@ __global__ void __launch_bounds__(256) k_gather_matrix(BellView M, const double*
   const int64_t r = t - (int64_t)s * nrow;
   const int64_t i = r / 3;
   const int c = (int)(r - 3 * i);
-  if (s >= M.nslot[i]) return;
   double v0 = 0.0, v1 = 0.0, v2 = 0.0;
-  if (is_bc[r]) {
+  if (s >= M.nslot[i]) {
+    // padding slot (column = own node): explicit zeros so the SpMV can run a fixed slot count
+  } else if (is_bc[r]) {
     const int64_t j = M.col[(int64_t)s * M.n_nodes + i];
     if (j == i) {
       v0 = (c == 0) ? 1.0 : 0.0;
