@@ -160,6 +160,35 @@ int cpfem_axpby(cpfem_ctx* ctx, void* stream, double alpha, const double* x, dou
 /* CSR export of the block-ELL matrix (host outputs: indptr n_dofs+1, indices/data nnz). */
 int cpfem_to_csr(cpfem_ctx* ctx, void* stream, const double* mat, int64_t* indptr, int32_t* indices, double* data);
 
+/* ---- adjoint (reverse mode) ----------------------------------------------------------
+   Replaces the backward pass JAX derives through the custom_jvp local solver
+   (models_copper.py:251-259) and the parameter VJP of implicit_vjp (solver.py:801-853). */
+typedef struct CpfemVjp {
+  const double* Pbar;      /* [9][n_qp] cotangent of P, or NULL */
+  const double* sbar_cell; /* [9][n_cells] cotangent of the cell-averaged Cauchy stress, or NULL */
+  const double* Fbar;      /* [9][n_qp] cotangent of Fp_inv_new, or NULL */
+  const double* gbar;      /* [n_slip][n_qp] cotangent of g_new, or NULL */
+  const double* slipbar;   /* [n_slip][n_qp] cotangent of slip_new, or NULL */
+  double* Hbar;            /* [9][n_qp]      out: cotangent of u_grad          (mask 1) */
+  double* Fpibar;          /* [9][n_qp]      out: cotangent of Fp_inv_old      (mask 2) */
+  double* gbar_old;        /* [n_slip][n_qp] out: cotangent of g_old           (mask 4) */
+  double* pbar;            /* [8][n_qp]      out: cotangent of xm, gss_a, h, t_sat, r, C11, C12, C44 (mask 8) */
+  double* Rbar;            /* [9][n_qp]      out: cotangent of R               (mask 16) */
+  int32_t mask;
+  int32_t accumulate;      /* 1: add into the outputs */
+} CpfemVjp;
+
+/* Per-point vector-Jacobian product through the converged local Newton (implicit-function
+   theorem; the slip cotangent passes through slip_old unchanged and is not an output). */
+int cpfem_local_vjp(cpfem_ctx* ctx, void* stream, const double* dofs, const CpfemState* state,
+                    const CpfemParams* params, const CpfemVjp* vjp);
+/* out (n_dofs) = B^T Hbar: the transpose of the dofs -> u_grad map (no Dirichlet handling). */
+int cpfem_ugrad_transpose(cpfem_ctx* ctx, void* stream, const double* Hbar, double* out);
+/* out [9][n_qp] = scale * JxW_q * grad(lambda)_q with lambda set to zero on Dirichlet dofs. */
+int cpfem_grad_field(cpfem_ctx* ctx, void* stream, const double* lambda, double scale, double* out);
+/* matT = A^T in the same block-ELL layout (for the adjoint solve A^T lambda = v). */
+int cpfem_transpose(cpfem_ctx* ctx, void* stream, const double* mat, double* matT);
+
 /* Kernel-class timing with HIP events on the launch stream (every `sample`-th launch of a
    class is timed, all are counted); enable resets the counters.  Classes (index):
    0 point update + tangent, 1 element stiffness/residual, 2 matrix gather, 3 residual

@@ -11,7 +11,8 @@ from .materials import (CALIB_COPPER, CALIB_STEEL304, CALIB_TANTALUM, COPPER, PR
 from .models import CrystalPlasticity
 from .problem import BlockEllMatrix, Problem
 from .solver import solver
+from .adjoint import ad_wrapper, implicit_vjp, simulate, LoadHistory
 
 __all__ = ["CrystalPlasticity", "Problem", "solver", "Mesh", "box_mesh", "get_meshio_cell_type", "Material",
            "BlockEllMatrix", "COPPER", "TANTALUM", "STEEL304", "CALIB_COPPER", "CALIB_TANTALUM",
-           "CALIB_STEEL304", "SENS_ROT", "PRESETS"]
+           "CALIB_STEEL304", "SENS_ROT", "PRESETS", "ad_wrapper", "implicit_vjp", "simulate", "LoadHistory"]

@@ -43,6 +43,15 @@ class CpfemParams(ctypes.Structure):
                 ("self_block", ctypes.c_int32)]
 
 
+class CpfemVjp(ctypes.Structure):
+    _fields_ = [("Pbar", vp), ("sbar_cell", vp), ("Fbar", vp), ("gbar", vp), ("slipbar", vp),
+                ("Hbar", vp), ("Fpibar", vp), ("gbar_old", vp), ("pbar", vp), ("Rbar", vp),
+                ("mask", ctypes.c_int32), ("accumulate", ctypes.c_int32)]
+
+
+VJP_H, VJP_FPI, VJP_G, VJP_MAT, VJP_R = 1, 2, 4, 8, 16
+
+
 class CpfemSolveInfo(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int32), ("outer", ctypes.c_int32),
                 ("residual", ctypes.c_double), ("b_norm", ctypes.c_double)]
@@ -71,6 +80,11 @@ _SIGS = {
     "cpfem_norm": (ctypes.c_int, [vp, vp, vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32)]),
     "cpfem_axpby": (ctypes.c_int, [vp, vp, ctypes.c_double, vp, ctypes.c_double, vp]),
     "cpfem_to_csr": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),
+    "cpfem_local_vjp": (ctypes.c_int, [vp, vp, vp, ctypes.POINTER(CpfemState), ctypes.POINTER(CpfemParams),
+                                       ctypes.POINTER(CpfemVjp)]),
+    "cpfem_ugrad_transpose": (ctypes.c_int, [vp, vp, vp, vp]),
+    "cpfem_grad_field": (ctypes.c_int, [vp, vp, vp, ctypes.c_double, vp]),
+    "cpfem_transpose": (ctypes.c_int, [vp, vp, vp, vp]),
     "cpfem_profile": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32]),
     "cpfem_profile_read": (ctypes.c_int, [vp, ctypes.c_int32, vp, vp, vp]),
     "cpfem_copy": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_int64]),

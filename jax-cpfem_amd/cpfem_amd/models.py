@@ -117,7 +117,10 @@ class CrystalPlasticity(Problem):
 
     # ------------------------------------------------------------------ state marshalling
     def _marshal(self, params, n_cells=None, n_qp=None):
-        """CpfemState + CpfemParams for a parameter list (no copies when already SoA)."""
+        """CpfemState + CpfemParams for a parameter list (no copies when already SoA).
+        While ``_override`` is set (whole-history adjoint driver) it is used instead."""
+        if getattr(self, "_override", None) is not None and n_cells is None:
+            return self._override
         nc = self.fes[0].num_cells if n_cells is None else n_cells
         Ns = self.num_slip_sys
         keep = []
@@ -207,7 +210,15 @@ class CrystalPlasticity(Problem):
         self.initial_guess = initial_sol
 
     def update_int_vars_gp(self, sol, params):
-        """New history at the converged solution (models_copper.py:273-282)."""
+        """New history at the converged solution (models_copper.py:273-282); differentiable
+        under torch autograd when sol or params require grad."""
+        from . import adjoint
+        if adjoint.differentiable(params, sol):
+            new = adjoint._UpdateFn.apply(self, sol, *params)
+            return list(new) + list(params[3:])
+        return self._update_raw(sol, params)
+
+    def _update_raw(self, sol, params):
         nc, Ns = self.fes[0].num_cells, self.num_slip_sys
         dofs = self.flatten(sol)
         st, pr, keep = self._marshal(params)
@@ -218,7 +229,14 @@ class CrystalPlasticity(Problem):
         return [Fn, gn, sn] + [p if isinstance(p, torch.Tensor) else to_device(p) for p in params[3:]]
 
     def compute_avg_stress(self, sol, params):
-        """JxW-weighted cell average of the Cauchy stress, (cells, 3, 3) (models_copper.py:297-319)."""
+        """JxW-weighted cell average of the Cauchy stress, (cells, 3, 3) (models_copper.py:297-319);
+        differentiable under torch autograd when sol or params require grad."""
+        from . import adjoint
+        if adjoint.differentiable(params, sol):
+            return adjoint._AvgStressFn.apply(self, sol, *params)
+        return self._compute_avg_stress_raw(sol, params)
+
+    def _compute_avg_stress_raw(self, sol, params):
         dofs = self.flatten(sol)
         st, pr, keep = self._marshal(params)
         sig = torch.empty(self.fes[0].num_cells, 3, 3, dtype=torch.float64, device=device())
@@ -239,6 +257,7 @@ class CrystalPlasticity(Problem):
         """Residual (and block-ELL Jacobian values) at dofs; newton_update + apply_bc_vec + get_A."""
         params = self.internal_vars if params is None else params
         st, pr, keep = self._marshal(params)
+        dofs = dofs.detach()
         res = torch.empty(self.ctx.n_dofs, dtype=torch.float64, device=device())
         mat = None
         if want_matrix:

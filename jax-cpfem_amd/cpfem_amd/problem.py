@@ -233,4 +233,4 @@ class Problem:
     def flatten(sol_list):
         if isinstance(sol_list, (list, tuple)):
             sol_list = sol_list[0]
-        return to_device(sol_list).reshape(-1).contiguous()
+        return to_device(sol_list).detach().reshape(-1).contiguous()

@@ -7,8 +7,9 @@ sequences the calls and reads back norms.
 
 solver_options keys (as in the reference): 'jax_solver': {'precond': bool}, 'umfpack_solver',
 'petsc_solver', 'custom_solver', 'initial_guess', 'tol' (1e-6), 'rel_tol' (1e-8),
-'line_search_flag' (False).  Added: 'precision': 'fp64' (default) | 'mixed' (FP32 Krylov with
-FP64 refinement), 'krylov_tol'/'krylov_atol' (1e-10), 'krylov_maxiter' (10000).
+'line_search_flag' (False).  Added: 'precision': 'fp64' (default) | 'mixed' (FP32 Krylov matrix
+with FP64 refinement), 'krylov_tol'/'krylov_atol' (1e-10), 'krylov_maxiter' (10000),
+'max_newton' (200; the reference loop has no cap and spins forever on a stagnating residual).
 'umfpack_solver' and 'petsc_solver' are served by the same device Krylov solver run to the
 same 1e-10 tolerance (no host direct solver exists on this path).
 """
@@ -126,7 +127,10 @@ def solver(problem, solver_options={}):
     rel_res_val = res_val / res_val_initial if res_val_initial != 0 else math.nan
     logger.debug(f"Before, l_2 res = {res_val}, relative l_2 res = {rel_res_val}")
     newton = 0
+    max_newton = int(solver_options.get("max_newton", 200))
     while (rel_res_val > rel_tol) and (res_val > tol):
+        if newton >= max_newton:
+            raise RuntimeError(f"Newton did not converge in {max_newton} iterations (l_2 res = {res_val})")
         dofs = linear_incremental_solver(problem, res, mat, dofs, solver_options)
         res, mat = problem.assemble(dofs)
         res_val = norm(problem, res, status)

@@ -96,7 +96,7 @@ struct cpfem_ctx {
   int32_t* d_cells = nullptr;
   double* d_sg = nullptr;
   double* d_jxw = nullptr;
-  int32_t *d_col = nullptr, *d_nslot = nullptr, *d_dslot = nullptr;
+  int32_t *d_col = nullptr, *d_nslot = nullptr, *d_dslot = nullptr, *d_tslot = nullptr;
   int32_t *d_cptr = nullptr, *d_clist = nullptr;
   int32_t *d_nptr = nullptr, *d_nlist = nullptr;
   uint8_t* d_isbc = nullptr;
@@ -271,6 +271,14 @@ static int build_maps(cpfem_ctx* c, const int32_t* cells) {
     for (int s = 0; s < maxs; ++s) c->h_col[(size_t)s * nn + i] = s < (int)v.size() ? v[s] : (int32_t)i;
     c->h_dslot[i] = (int32_t)(std::lower_bound(v.begin(), v.end(), (int32_t)i) - v.begin());
   }
+  // transpose map: slot s of row j points at node i; tslot gives the slot of node j in row i
+  std::vector<int32_t> tslot((size_t)maxs * nn, 0);
+  for (int64_t j = 0; j < nn; ++j)
+    for (int s = 0; s < (int)nb[j].size(); ++s) {
+      const int32_t i = nb[j][s];
+      const auto& v = nb[i];
+      tslot[(size_t)s * nn + j] = (int32_t)(std::lower_bound(v.begin(), v.end(), (int32_t)j) - v.begin());
+    }
   // (node i, slot s) -> element blocks (cell*64 + a*8 + b), ascending cell
   const int64_t nkey = (int64_t)maxs * nn;
   std::vector<int32_t> cptr(nkey + 1, 0), clist((size_t)nc * 64);
@@ -297,8 +305,9 @@ static int build_maps(cpfem_ctx* c, const int32_t* cells) {
   if ((rc = dalloc(&c->d_nptr, nptr.size())) || (rc = dalloc(&c->d_nlist, nlist.size())) ||
       (rc = dalloc(&c->d_cptr, cptr.size())) || (rc = dalloc(&c->d_clist, clist.size())) ||
       (rc = dalloc(&c->d_col, c->h_col.size())) || (rc = dalloc(&c->d_nslot, c->h_nslot.size())) ||
-      (rc = dalloc(&c->d_dslot, c->h_dslot.size())))
+      (rc = dalloc(&c->d_dslot, c->h_dslot.size())) || (rc = dalloc(&c->d_tslot, tslot.size())))
     return rc;
+  HIPCHK(hipMemcpy(c->d_tslot, tslot.data(), tslot.size() * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_nptr, nptr.data(), nptr.size() * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_nlist, nlist.data(), nlist.size() * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_cptr, cptr.data(), cptr.size() * 4, hipMemcpyHostToDevice));
@@ -521,7 +530,7 @@ void cpfem_destroy(cpfem_ctx* c) {
   if (!c) return;
   (void)hipDeviceSynchronize();
   dfree(c->d_cells); dfree(c->d_sg); dfree(c->d_jxw);
-  dfree(c->d_col); dfree(c->d_nslot); dfree(c->d_dslot);
+  dfree(c->d_col); dfree(c->d_nslot); dfree(c->d_dslot); dfree(c->d_tslot);
   dfree(c->d_cptr); dfree(c->d_clist); dfree(c->d_nptr); dfree(c->d_nlist);
   dfree(c->d_isbc); dfree(c->d_nobc); dfree(c->d_ubar);
   dfree(c->d_wP); dfree(c->d_wA); dfree(c->d_KE); dfree(c->d_RE); dfree(c->d_sig); dfree(c->d_status);
@@ -774,6 +783,60 @@ int cpfem_to_csr(cpfem_ctx* c, void* stream, const double* mat, int64_t* indptr,
     }
     indptr[r + 1] = nz;
   }
+  return CPFEM_OK;
+}
+
+int cpfem_local_vjp(cpfem_ctx* c, void* stream, const double* dofs, const CpfemState* state, const CpfemParams* params,
+                    const CpfemVjp* vjp) {
+  if (!c || !dofs || !vjp) return fail(CPFEM_EBADARG, "null argument");
+  int rc;
+  if ((rc = check_state(c, state, params))) return rc;
+  VjpArgs va;
+  va.Pbar = vjp->Pbar;
+  va.sbar_cell = vjp->sbar_cell;
+  va.Fbar = vjp->Fbar;
+  va.gbar = vjp->gbar;
+  va.slipbar = vjp->slipbar;
+  va.Hbar = vjp->Hbar;
+  va.Fpibar = vjp->Fpibar;
+  va.gbar_old = vjp->gbar_old;
+  va.pbar = vjp->pbar;
+  va.Rbar = vjp->Rbar;
+  va.mask = vjp->mask;
+  va.accumulate = vjp->accumulate;
+  hipStream_t st = (hipStream_t)stream;
+  const Geom G = c->geom();
+  const StateView sv = state_view(state, c->n_qp);
+  const MatView mv = mat_view(params);
+  const SolveCfg cfg = solve_cfg(params);
+  if (c->n_slip == 12)
+    launch_local_vjp<12>(G, dofs, sv, mv, c->tab12, cfg, params->self_block, va, st);
+  else
+    launch_local_vjp<24>(G, dofs, sv, mv, c->tab24, cfg, params->self_block, va, st);
+  HIPCHK(hipGetLastError());
+  return CPFEM_OK;
+}
+
+int cpfem_ugrad_transpose(cpfem_ctx* c, void* stream, const double* Hbar, double* out) {
+  if (!c || !Hbar || !out) return fail(CPFEM_EBADARG, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  launch_element_K(c->geom(), Hbar, nullptr, c->d_KE, c->d_RE, st);
+  launch_gather_residual(c->n_nodes, c->d_RE, c->d_nptr, c->d_nlist, c->d_nobc, c->d_ubar, Hbar, out, st);
+  HIPCHK(hipGetLastError());
+  return CPFEM_OK;
+}
+
+int cpfem_grad_field(cpfem_ctx* c, void* stream, const double* lam, double scale, double* out) {
+  if (!c || !lam || !out) return fail(CPFEM_EBADARG, "null argument");
+  launch_grad_field(c->geom(), lam, c->d_isbc, scale, out, (hipStream_t)stream);
+  HIPCHK(hipGetLastError());
+  return CPFEM_OK;
+}
+
+int cpfem_transpose(cpfem_ctx* c, void* stream, const double* mat, double* matT) {
+  if (!c || !mat || !matT || mat == matT) return fail(CPFEM_EBADARG, "bad transpose arguments");
+  launch_transpose(c->bell(), c->d_tslot, mat, matT, (hipStream_t)stream);
+  HIPCHK(hipGetLastError());
   return CPFEM_OK;
 }
 

@@ -72,6 +72,28 @@ void launch_gather_residual(int64_t n_nodes, const double* RE, const int32_t* np
 void launch_cell_average(const Geom& G, const double* sig, double* out, hipStream_t st);
 void launch_bc_x0(int64_t n, const uint8_t* is_bc, const double* ubar, const double* u, double* x0, hipStream_t st);
 
+// ---------------------------------------------------------------- adjoint
+struct VjpArgs {
+  const double* Pbar;       // [9][Q] or null
+  const double* sbar_cell;  // [9][n_cells] or null (cotangent of the cell-averaged Cauchy stress)
+  const double* Fbar;       // [9][Q] or null (cotangent of Fp_inv_new)
+  const double* gbar;       // [N][Q] or null (cotangent of g_new)
+  const double* slipbar;    // [N][Q] or null (cotangent of slip_new)
+  double* Hbar;             // [9][Q]
+  double* Fpibar;           // [9][Q]
+  double* gbar_old;         // [N][Q]
+  double* pbar;             // [8][Q] (xm, gss_a, h, t_sat, r, C11, C12, C44)
+  double* Rbar;             // [9][Q]
+  int mask;
+  int accumulate;
+};
+
+template <int N>
+void launch_local_vjp(const Geom& G, const double* u, const StateView& sv, const MatView& mv, const SlipTable<N>& tab,
+                      const SolveCfg& cfg, int self_block, const VjpArgs& va, hipStream_t st);
+void launch_grad_field(const Geom& G, const double* lam, const uint8_t* is_bc, double scale_by_w, double* out, hipStream_t st);
+void launch_transpose(const BellView& M, const int32_t* tslot, const double* A, double* At, hipStream_t st);
+
 // ---------------------------------------------------------------- profiling
 // Optional per-kernel-class timing with HIP events recorded on the launch stream around
 // every `sample`-th launch of a class; the rest are only counted.

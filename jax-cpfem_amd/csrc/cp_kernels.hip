@@ -58,8 +58,9 @@ __device__ __forceinline__ void load_point(const StateView& sv, int64_t q, doubl
 constexpr int QBS = 128;  // threads per block of the point kernels (LDS: 2 * N * QBS doubles)
 
 // ---------------------------------------------------------------------------------
-// Quadrature-point kernel of the assembly: H from dofs, local Newton, JxW-weighted P and
-// (if wA) the consistent tangent, written SoA ([9][Q], [81][Q]).
+// Quadrature-point kernel of the assembly: H from dofs, local Newton, JxW-weighted P (SoA
+// [9][Q]) and, if wA, the JxW-weighted consistent tangent stored per point ([Q][81]: the
+// element kernel's 8 threads of a cell then read whole cache lines together).
 template <int N>
 __global__ void __launch_bounds__(QBS) k_element_qp(Geom G, const double* __restrict__ u, StateView sv, MatView mv,
                                                    SlipTable<N> tab, SolveCfg cfg, double* __restrict__ wP,
@@ -86,9 +87,10 @@ __global__ void __launch_bounds__(QBS) k_element_qp(Geom G, const double* __rest
   for (int i = 0; i < 9; ++i) wP[i * G.n_qp + q] = w * P[i];
   if (wA) {
     const int64_t Q = G.n_qp;
+    (void)Q;
     tangent<N>(pt, tab, F, Fpi, s, [&](int m9, const double* dP) {
 #pragma unroll
-      for (int i = 0; i < 9; ++i) wA[(9 * i + m9) * Q + q] = w * dP[i];
+      for (int i = 0; i < 9; ++i) wA[q * 81 + 9 * i + m9] = w * dP[i];
     });
   }
   if (iters) iters[q] = it;
@@ -124,6 +126,7 @@ __global__ void __launch_bounds__(256) k_element_K(Geom G, const double* __restr
       for (int j = 0; j < 3; ++j) s += wP[(3 * i + j) * G.n_qp + qp] * ga[j];
       re[i] += s;
     }
+    const double* Aq = wA + qp * 81;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       double Z[9];  // Z[k][l] = sum_j ga_j A[(i j),(k l)]
@@ -131,7 +134,7 @@ __global__ void __launch_bounds__(256) k_element_K(Geom G, const double* __restr
       for (int kl = 0; kl < 9; ++kl) {
         double s = 0.0;
 #pragma unroll
-        for (int j = 0; j < 3; ++j) s += ga[j] * wA[(9 * (3 * i + j) + kl) * G.n_qp + qp];
+        for (int j = 0; j < 3; ++j) s += ga[j] * Aq[9 * (3 * i + j) + kl];
         Z[kl] = s;
       }
 #pragma unroll
