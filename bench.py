@@ -32,14 +32,17 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def spmv_bytes(prob, value_bytes=8, vec_bytes=8):
-    """Algorithmic HBM bytes of one Krylov SpMV launch (k_kq / k_kt): block values,
-    one column index per 3x3 block, slot counts, the multiplied vector, one extra vector read
-    (rhat or s) and the result write."""
+def spmv_bytes(prob, value_bytes=8):
+    """Algorithmic HBM bytes of one Krylov SpMV launch (k_kpq / k_kst, equal counts): the
+    block values (value_bytes each), one int32 column index per 3x3 block (plus the slot
+    counts, read unless every row is padded to 27 slots; padding is not counted), the two FP64 vectors the
+    operand is formed from (read once each), and five FP64 row vectors (k_kpq: reads rhat and
+    the Jacobi vector, writes phat, q, M q; k_kst: reads r, q, writes s, shat, t)."""
     nslot, _ = prob.ctx.pattern()
     nnzb = int(nslot.sum())
     n = prob.ctx.n_dofs
-    return 9 * value_bytes * nnzb + 4 * nnzb + 4 * prob.ctx.n_nodes + 3 * vec_bytes * n, nnzb
+    idx = 4 * nnzb + (0 if prob.ctx.n_slots == 27 else 4 * prob.ctx.n_nodes)
+    return 9 * value_bytes * nnzb + idx + 7 * 8 * n, nnzb
 
 
 def cpu_baseline(prob, w, params, sol, per_step, sample_cells=512, krylov_iters=16):
@@ -164,12 +167,12 @@ def main():
     Q = w.n_qp
     value = Q * K / T
     vb = 4 if args.precision == "mixed" else 8
-    sb, nnzb = spmv_bytes(prob, value_bytes=vb, vec_bytes=vb)
+    sb, nnzb = spmv_bytes(prob, value_bytes=vb)
     roof = None
     if dom == "krylov_spmv" and prof[dom]["avg_ms"]:
         ach = sb / (prof[dom]["avg_ms"] * 1e-3) / 1e9
         roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4),
-                    traffic=None, kernel="k_kq/k_kt (Krylov SpMV with fused dots)",
+                    traffic=None, kernel="k_kpq/k_kst (BiCGSTAB SpMV with fused p/s updates and dots)",
                     bytes_per_launch=sb, avg_launch_us=round(prof[dom]["avg_ms"] * 1e3, 2))
     elif prof[dom]["avg_ms"]:
         roof = dict(bound="mfma" if False else "hbm", achieved=None, peak=None, unit=None, frac=None, traffic=None,

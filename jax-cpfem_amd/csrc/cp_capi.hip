@@ -113,7 +113,7 @@ struct cpfem_ctx {
   // Krylov
   KrylovScalars* d_ksc = nullptr;
   double* d_kpartial = nullptr;  // [PS_COUNT][krylov blocks]
-  double* kv64[10] = {};  // r rhat p q phat s shat t invd d
+  double* kv64[12] = {};  // r rhat pp q phat s shat t invd (spare) rh qh
   float* d_mat32 = nullptr;
   double *d_r64 = nullptr, *d_tmp64 = nullptr;
   // pinned host staging
@@ -371,8 +371,8 @@ static int ensure_krylov(cpfem_ctx* c, bool mixed) {
   int rc;
   const int64_t n = c->n_dofs;
   if (!c->d_ksc) {
-    if ((rc = dalloc(&c->d_ksc, 1)) || (rc = dalloc(&c->d_kpartial, (size_t)8 * krylov_blocks(n)))) return rc;
-    for (int i = 0; i < 10; ++i)
+    if ((rc = dalloc(&c->d_ksc, 1)) || (rc = dalloc(&c->d_kpartial, (size_t)8 * krylov_stride(n)))) return rc;
+    for (int i = 0; i < 12; ++i)
       if ((rc = dalloc(&c->kv64[i], n))) return rc;
     if ((rc = dalloc(&c->d_r64, n)) || (rc = dalloc(&c->d_tmp64, n))) return rc;
     HIPCHK(hipMemset(c->d_ksc, 0, sizeof(KrylovScalars)));
@@ -396,13 +396,15 @@ static KVec kvec(double** v, double* x) {
   k.x = x;
   k.r = v[0];
   k.rhat = v[1];
-  k.p = v[2];
+  k.pp = v[2];
   k.q = v[3];
   k.phat = v[4];
   k.s = v[5];
   k.shat = v[6];
   k.t = v[7];
   k.invd = v[8];
+  k.rh = v[10];
+  k.qh = v[11];
   return k;
 }
 
@@ -423,6 +425,8 @@ static int run_bicgstab(cpfem_ctx* c, hipStream_t st, const TM* vals, const doub
   KrylovWork w;
   w.n = c->n_dofs;
   w.nb = krylov_blocks(c->n_dofs);
+  w.nbs = krylov_spmv_blocks(c->n_dofs);
+  w.ps = krylov_stride(c->n_dofs);
   w.partial = c->d_kpartial;
   w.sc = c->d_ksc;
   KVec v = kvec(c->kv64, x);
@@ -537,7 +541,7 @@ void cpfem_destroy(cpfem_ctx* c) {
   dfree(c->d_partial); dfree(c->d_scalar); dfree(c->d_counter);
   dfree(c->d_ksc);
   dfree(c->d_kpartial);
-  for (int i = 0; i < 10; ++i) dfree(c->kv64[i]);
+  for (int i = 0; i < 12; ++i) dfree(c->kv64[i]);
   dfree(c->d_mat32); dfree(c->d_r64); dfree(c->d_tmp64);
   if (c->h_ksc) (void)hipHostFree(c->h_ksc);
   if (c->h_dbl) (void)hipHostFree(c->h_dbl);

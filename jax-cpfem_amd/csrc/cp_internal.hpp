@@ -143,13 +143,18 @@ struct KrylovScalars {
 
 struct KrylovWork {
   int64_t n;
-  int nb;            // blocks of the vector / SpMV kernels
-  double* partial;   // [4][nb]
+  int nb;            // blocks of the vector kernels (one row per thread, grid-stride)
+  int nbs;           // blocks of the SpMV kernels (64 rows x 4 slot parts per block)
+  int ps;            // partial-sum stride: max(nb, nbs); unused entries hold zeros
+  double* partial;   // [slot][ps]
   KrylovScalars* sc;
 };
 
+// BiCGSTAB vectors.  Besides JAX's x, r, rhat, q, phat = M p, s, shat = M s, t the
+// iteration keeps rh = M r, qh = M q and pp = M (p - omega q), so that the next phat =
+// rh + beta pp and shat = rh - alpha qh can be formed inside the SpMV kernels.
 struct KVec {
-  double *x, *r, *rhat, *p, *q, *phat, *s, *shat, *t;
+  double *x, *r, *rhat, *pp, *q, *phat, *s, *shat, *t, *rh, *qh;
   const double* invd;
 };
 
@@ -158,6 +163,8 @@ void krylov_init(const BellView& M, const TM* vals, const double* b, KVec v, Kry
 template <typename TM>
 void krylov_iteration(const BellView& M, const TM* vals, KVec v, KrylovWork w, int par, hipStream_t st, Prof* prof);
 int krylov_blocks(int64_t n);
+int krylov_spmv_blocks(int64_t n);
+inline int krylov_stride(int64_t n) { return krylov_blocks(n) > krylov_spmv_blocks(n) ? krylov_blocks(n) : krylov_spmv_blocks(n); }
 void krylov_replace(const BellView& M, const double* vals64, const double* b, KVec v, KrylovWork w, hipStream_t st);
 void launch_spmv(const BellView& M, const double* vals, const double* x, double* y, hipStream_t st);
 void launch_inv_diag(const BellView& M, const double* vals, const int32_t* dslot, double* invd, int precond, hipStream_t st);

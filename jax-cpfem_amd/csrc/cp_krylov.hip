@@ -13,11 +13,16 @@
 //     x += alpha phat + omega shat (alpha phat only if <s,s> < atol2); r = s - omega t (or s)
 //     k = -11 on alpha/omega = 0, -10 on rho_ = 0, else k + 1
 //
-// Design: five kernels per iteration; every dot product is fused into the kernel that
-// produces its operand and is finished by the last arriving block (fixed-order sum of
-// per-block partials, so results are bitwise reproducible run to run).  All scalars stay on
-// the device; each kernel returns at once when the device-side `done` flag is set, so the
-// host can enqueue iterations in batches and poll rarely.
+// Design: three kernels per iteration.  p is never stored: with rh = M r, qh = M q and
+// pp = M (p - omega q) kept per row (M is diagonal), phat = rh + beta pp and
+// shat = rh - alpha qh are formed on the fly wherever the SpMV gathers them, so the p and s
+// updates ride inside the two SpMV kernels and only the x/r update is a kernel of its own.
+// (Same recurrence as JAX's in exact arithmetic; the Jacobi scaling is applied before instead
+// of after the vector sums, which moves results by rounding only.)  Every dot product is
+// fused into the kernel producing its operand as per-block partial sums, which the consuming
+// kernel adds in a fixed order (bitwise reproducible run to run, no atomics).  All scalars
+// stay on the device; each kernel returns at once when the device-side `done` flag is set,
+// so the host enqueues iterations in batches and polls rarely.
 #include <algorithm>
 
 #include "cp_internal.hpp"
@@ -148,67 +153,91 @@ __global__ void __launch_bounds__(KBS) k_inv_diag(BellView M, const double* __re
 }
 
 // ---------------------------------------------------------------------------------
-// BiCGSTAB kernels.  Grid: KrylovWork::nb blocks (<= KMAXB) with grid-stride loops.  Each
+// BiCGSTAB kernels.  Vector kernels: KrylovWork::nb blocks (<= KMAXB) of KBS threads with
+// grid-stride loops.  SpMV kernels: KrylovWork::nbs blocks of SBS threads (below).  Each
 // reducing kernel writes one partial per block into its slot of w.partial; the kernel that
-// consumes the sum adds the nb partials itself, in every block and in the same order, so no
-// atomics are needed and every block sees the bitwise-identical scalar.  Scalars that carry
-// over to the next iteration live in two parity slots: iteration j writes slot (j & 1) and
-// reads slot ((j + 1) & 1), which the host passes as `par`.
+// consumes the sum adds the slot's ps entries itself, in every block and in the same order,
+// so every block sees the bitwise-identical scalar.  Scalars that carry over to the next
+// iteration live in two parity slots: iteration j writes slot (j & 1) and reads slot
+// ((j + 1) & 1), which the host passes as `par`.
 enum { PS_RR = 0, PS_RHR, PS_RQ, PS_SS, PS_TS, PS_TT, PS_BB, PS_COUNT };
 
-__device__ __forceinline__ void put_partial(double v, double* partial, int slot, int nb, double* sh) {
+// Writes this block's NSL sums into entry blockIdx.x of their slots; a grid smaller than
+// the stride ps also clears the entries no block of it owns, so every consumer sums exactly
+// ps values.  sh: NSL doubles per wave of the block.
+template <int BS, int NSL>
+__device__ __forceinline__ void put_partials(const double (&v)[NSL], const int (&slot)[NSL], double* partial, int ps, double* sh) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  v = wave_sum(v);
-  if (lane == 0) sh[wid] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double s = 0.0;
 #pragma unroll
-    for (int i = 0; i < KBS / 64; ++i) s += sh[i];
-    partial[slot * nb + blockIdx.x] = s;
+  for (int j = 0; j < NSL; ++j) {
+    const double t = wave_sum(v[j]);
+    if (lane == 0) sh[j * (BS / 64) + wid] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < NSL) {
+    const int j = threadIdx.x;
+    double t = 0.0;
+#pragma unroll
+    for (int i = 0; i < BS / 64; ++i) t += sh[j * (BS / 64) + i];
+    partial[slot[j] * ps + blockIdx.x] = t;
+  }
+  for (int i = blockIdx.x + (threadIdx.x + 1) * gridDim.x; i < ps; i += BS * gridDim.x) {
+#pragma unroll
+    for (int j = 0; j < NSL; ++j) partial[slot[j] * ps + i] = 0.0;
   }
 }
 
-// Sum of one partial slot, computed by the whole block; returned in every thread.
-__device__ __forceinline__ double get_sum(const double* partial, int slot, int nb, double* sh) {
-  double s = 0.0;
-  for (int i = threadIdx.x; i < nb; i += KBS) s += partial[slot * nb + i];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  s = wave_sum(s);
-  if (lane == 0) sh[wid] = s;
-  __syncthreads();
-  double t = 0.0;
+// Sums of NSL partial slots, computed by the whole block (all slots' loads in one pass);
+// returned in every thread, identical in every block.
+template <int BS, int NSL>
+__device__ __forceinline__ void get_sums(const double* partial, const int (&slot)[NSL], int ps, double* sh, double (&out)[NSL]) {
+  double s[NSL];
 #pragma unroll
-  for (int i = 0; i < KBS / 64; ++i) t += sh[i];
+  for (int j = 0; j < NSL; ++j) s[j] = 0.0;
+  for (int i = threadIdx.x; i < ps; i += BS) {
+#pragma unroll
+    for (int j = 0; j < NSL; ++j) s[j] += partial[slot[j] * ps + i];
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < NSL; ++j) {
+    const double t = wave_sum(s[j]);
+    if (lane == 0) sh[j * (BS / 64) + wid] = t;
+  }
   __syncthreads();
-  return t;
+#pragma unroll
+  for (int j = 0; j < NSL; ++j) {
+    double t = 0.0;
+#pragma unroll
+    for (int i = 0; i < BS / 64; ++i) t += sh[j * (BS / 64) + i];
+    out[j] = t;
+  }
+  __syncthreads();
 }
 
 template <typename TM>
 __global__ void __launch_bounds__(KBS) k_kinit(BellView M, const TM* __restrict__ vals, const double* __restrict__ b, KVec v,
                                               KrylovWork w) {
-  __shared__ double sh[KBS / 64];
+  __shared__ double sh[3 * (KBS / 64)];
   double rr = 0.0, bb = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * KBS + threadIdx.x; i < w.n; i += (int64_t)w.nb * KBS) {
     const double r = b[i] - spmv_row<TM>(M, vals, v.x, i);
     v.r[i] = r;
     v.rhat[i] = r;
-    v.p[i] = r;
-    v.q[i] = r;
+    v.rh[i] = r * v.invd[i];
+    v.pp[i] = 0.0;  // M (p0 - omega0 q0) with p0 = q0 = r0, omega0 = 1
     rr += r * r;
     bb += b[i] * b[i];
   }
-  put_partial(rr, w.partial, PS_RR, w.nb, sh);
-  __syncthreads();
-  put_partial(rr, w.partial, PS_RHR, w.nb, sh);
-  __syncthreads();
-  put_partial(bb, w.partial, PS_BB, w.nb, sh);
+  put_partials<KBS, 3>({rr, rr, bb}, {PS_RR, PS_RHR, PS_BB}, w.partial, w.ps, sh);
 }
 
 // Single block: atol2 and the "previous iteration" scalars seen by iteration 0.
 __global__ void __launch_bounds__(KBS) k_kinit2(KrylovWork w) {
   __shared__ double sh[KBS / 64];
-  const double bb = get_sum(w.partial, PS_BB, w.nb, sh);
+  double bbs[1];
+  get_sums<KBS, 1>(w.partial, {PS_BB}, w.ps, sh, bbs);
+  const double bb = bbs[0];
   if (threadIdx.x == 0) {
     KrylovScalars* sc = w.sc;
     sc->bs = bb;
@@ -220,123 +249,271 @@ __global__ void __launch_bounds__(KBS) k_kinit2(KrylovWork w) {
   }
 }
 
-// Start of iteration: convergence test of the previous one, then p and phat.
-__global__ void __launch_bounds__(KBS) k_kp(KVec v, KrylovWork w, int par) {
-  __shared__ double sh[KBS / 64];
-  KrylovScalars* sc = w.sc;
-  if (sc->done) return;
-  const int pv = par ^ 1;
-  const double rs = get_sum(w.partial, PS_RR, w.nb, sh);
-  const double rho_ = get_sum(w.partial, PS_RHR, w.nb, sh);
-  const double rho = sc->rho2[pv], alpha = sc->alpha2[pv], omega = sc->omega2[pv];
-  int k = (omega == 0.0 || alpha == 0.0) ? -11 : sc->k2[pv] + 1;
-  if (rho == 0.0) k = -10;
-  const bool go = (rs > sc->atol2) && (k < sc->maxiter) && (k >= 0);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    sc->k2[par] = k;
-    sc->rho2[par] = rho_;
-    sc->rs = rs;
-    sc->k = k;
-    if (!go) sc->done = 1;
-  }
-  if (!go) return;
-  const double beta = rho_ / rho * alpha / omega;
-  for (int64_t i = (int64_t)blockIdx.x * KBS + threadIdx.x; i < w.n; i += (int64_t)w.nb * KBS) {
-    const double p = v.r[i] + beta * (v.p[i] - omega * v.q[i]);
-    v.p[i] = p;
-    v.phat[i] = p * v.invd[i];
-  }
-}
+// ---- SpMV of the BiCGSTAB iteration.  A block of SBS = 1024 threads takes 4 groups of 64
+// consecutive rows (one row per lane) and splits each row's node slots over 4 waves (the
+// wave of part p takes slots p, p+4, ...), so a launch keeps four times as many waves in
+// flight as a thread-per-row SpMV while every wave still reads the column-major values
+// coalesced.  The four partial row sums meet in LDS and the part-0 wave adds them in fixed
+// order (bitwise reproducible).  The gathered operand is a functor of the column index, so
+// the p and s updates of the iteration are folded into the gather.
+//
+// Latency: the column indices and block values do not depend on the iteration's scalars, so
+// on 27-slot meshes a block issues them for its first row group before it reduces the
+// partial sums the scalars come from; the matrix stream and the reduction then share one
+// memory round trip instead of following each other.
+constexpr int SP_ROWS = 64;
+constexpr int SP_PARTS = 4;
+constexpr int SP_GROUPS = 4;
+constexpr int SP_BLOCK_ROWS = SP_ROWS * SP_GROUPS;
+constexpr int SBS = SP_ROWS * SP_PARTS * SP_GROUPS;
+constexpr int SP_MAXB = KMAXB;
+constexpr int SP_NS = (27 + SP_PARTS - 1) / SP_PARTS;  // slots per part on 27-slot meshes
 
+struct SpmvLane {
+  int lane, part, group;
+  __device__ SpmvLane() : lane(threadIdx.x & 63), part((threadIdx.x >> 6) / SP_GROUPS), group((threadIdx.x >> 6) % SP_GROUPS) {}
+};
+
+// One part's share of one row on a 27-slot mesh: column nodes and 3x3-block row values.
 template <typename TM>
-__global__ void __launch_bounds__(KBS) k_kq(BellView M, const TM* __restrict__ vals, KVec v, KrylovWork w) {
-  __shared__ double sh[KBS / 64];
-  if (w.sc->done) return;
-  double d = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * KBS + threadIdx.x; i < w.n; i += (int64_t)w.nb * KBS) {
-    const double q = spmv_row<TM>(M, vals, v.phat, i);
-    v.q[i] = q;
-    d += v.rhat[i] * q;
+struct SlotShare {
+  int32_t j[SP_NS];
+  TM a[SP_NS][3];
+  __device__ __forceinline__ void load_cols(const BellView& M, int64_t r, int part) {
+    const int64_t i = r / 3;
+#pragma unroll
+    for (int k = 0; k < SP_NS; ++k) {
+      const int s = part + SP_PARTS * k;
+      j[k] = s < 27 ? M.col[(int64_t)s * M.n_nodes + i] : 0;
+    }
   }
-  put_partial(d, w.partial, PS_RQ, w.nb, sh);
+  __device__ __forceinline__ void load_vals(const BellView& M, const TM* __restrict__ vals, int64_t r, int part) {
+    const int64_t nrow = 3 * M.n_nodes;
+#pragma unroll
+    for (int k = 0; k < SP_NS; ++k) {
+      const int s = part + SP_PARTS * k;
+      const TM* v = vals + (3 * (int64_t)s) * nrow + r;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) a[k][c] = s < 27 ? v[c * nrow] : TM(0);
+    }
+  }
+  template <typename XF>
+  __device__ __forceinline__ double dot(XF xf, int part) const {
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < SP_NS; ++k) {
+      if (part + SP_PARTS * k < 27) {
+        const int64_t jj = 3 * (int64_t)j[k];
+        acc += (double)a[k][0] * xf(jj) + (double)a[k][1] * xf(jj + 1) + (double)a[k][2] * xf(jj + 2);
+      }
+    }
+    return acc;
+  }
+};
+
+// Any slot count (meshes that are not structured hex boxes).
+template <typename TM, typename XF>
+__device__ __forceinline__ double spmv_part_any(const BellView& M, const TM* __restrict__ vals, XF xf, int64_t r, int part) {
+  const int64_t nrow = 3 * M.n_nodes;
+  const int64_t i = r / 3;
+  const int ns = M.nslot[i];
+  double acc = 0.0;
+  for (int s = part; s < ns; s += SP_PARTS) {
+    const int64_t j = M.col[(int64_t)s * M.n_nodes + i];
+    const TM* v = vals + (3 * (int64_t)s) * nrow + r;
+    acc += (double)v[0] * xf(3 * j) + (double)v[nrow] * xf(3 * j + 1) + (double)v[2 * nrow] * xf(3 * j + 2);
+  }
+  return acc;
 }
 
-__global__ void __launch_bounds__(KBS) k_ks(KVec v, KrylovWork w, int par) {
-  __shared__ double sh[KBS / 64];
-  KrylovScalars* sc = w.sc;
-  if (sc->done) return;
-  const double alpha = sc->rho2[par] / get_sum(w.partial, PS_RQ, w.nb, sh);
-  if (blockIdx.x == 0 && threadIdx.x == 0) sc->alpha2[par] = alpha;
-  double d = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * KBS + threadIdx.x; i < w.n; i += (int64_t)w.nb * KBS) {
-    const double s = v.r[i] - alpha * v.q[i];
-    v.s[i] = s;
-    v.shat[i] = s * v.invd[i];
-    d += s * s;
-  }
-  put_partial(d, w.partial, PS_SS, w.nb, sh);
-}
-
-template <typename TM>
-__global__ void __launch_bounds__(KBS) k_kt(BellView M, const TM* __restrict__ vals, KVec v, KrylovWork w) {
-  __shared__ double sh[KBS / 64];
-  if (w.sc->done) return;
-  double ts = 0.0, tt = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * KBS + threadIdx.x; i < w.n; i += (int64_t)w.nb * KBS) {
-    const double t = spmv_row<TM>(M, vals, v.shat, i);
-    v.t[i] = t;
-    ts += t * v.s[i];
-    tt += t * t;
-  }
-  put_partial(ts, w.partial, PS_TS, w.nb, sh);
+// Adds the parts' row sums: complete in the part-0 waves.  red: (SP_PARTS - 1) * SBS / SP_PARTS doubles.
+__device__ __forceinline__ double row_sum(double acc, const SpmvLane& t, double* red) {
+  if (t.part) red[((t.part - 1) * SP_GROUPS + t.group) * SP_ROWS + t.lane] = acc;
   __syncthreads();
-  put_partial(tt, w.partial, PS_TT, w.nb, sh);
+  if (t.part == 0) {
+#pragma unroll
+    for (int p = 1; p < SP_PARTS; ++p) acc += red[((p - 1) * SP_GROUPS + t.group) * SP_ROWS + t.lane];
+  }
+  __syncthreads();
+  return acc;
 }
 
-__global__ void __launch_bounds__(KBS) k_kxr(KVec v, KrylovWork w, int par) {
-  __shared__ double sh[KBS / 64];
+// Row-group loop of the two SpMV kernels.  pre(): scalars (called once, after the first
+// group's matrix loads are in flight; returns false to stop, uniformly over the grid).
+// make_xf(): the gathered operand.  own(r, y): row epilogue in the part-0 waves.  Returns
+// false if pre() stopped the sweep.
+template <typename TM, typename Pre, typename MakeXF, typename Own>
+__device__ __forceinline__ bool spmv_sweep(const BellView& M, const TM* __restrict__ vals, int64_t n, double* red, Pre pre,
+                                           MakeXF make_xf, Own own) {
+  const SpmvLane t;
+  const bool s27 = M.n_slots == 27;
+  int64_t r0 = (int64_t)blockIdx.x * SP_BLOCK_ROWS;
+  int64_t r = r0 + t.group * SP_ROWS + t.lane;
+  SlotShare<TM> sh27;
+  if (s27 && r < n) {
+    sh27.load_cols(M, r, t.part);
+    sh27.load_vals(M, vals, r, t.part);
+  }
+  if (!pre()) return false;
+  const auto xf = make_xf();
+  for (; r0 < n; r0 += (int64_t)gridDim.x * SP_BLOCK_ROWS) {
+    r = r0 + t.group * SP_ROWS + t.lane;
+    double acc = 0.0;
+    if (r < n) {
+      if (s27) {
+        if (r0 != (int64_t)blockIdx.x * SP_BLOCK_ROWS) {
+          sh27.load_cols(M, r, t.part);
+          sh27.load_vals(M, vals, r, t.part);
+        }
+        acc = sh27.dot(xf, t.part);
+      } else {
+        acc = spmv_part_any<TM>(M, vals, xf, r, t.part);
+      }
+    }
+    acc = row_sum(acc, t, red);
+    if (t.part == 0 && r < n) own(r, acc);
+  }
+  return true;
+}
+
+// Start of iteration j: convergence test of iteration j-1, beta, then
+// q = A phat with phat = rh + beta pp gathered on the fly; stores phat, q, qh and <rhat,q>.
+template <typename TM>
+__global__ void __launch_bounds__(SBS) k_kpq(BellView M, const TM* __restrict__ vals, KVec v, KrylovWork w, int par) {
+  __shared__ double sh[2 * (SBS / 64)];
+  __shared__ double red[(SP_PARTS - 1) * SP_GROUPS * SP_ROWS];
   KrylovScalars* sc = w.sc;
   if (sc->done) return;
-  const double ss = get_sum(w.partial, PS_SS, w.nb, sh);
-  const double omega = get_sum(w.partial, PS_TS, w.nb, sh) / get_sum(w.partial, PS_TT, w.nb, sh);
+  double beta = 0.0;
+  auto pre = [&]() {
+    const int pv = par ^ 1;
+    double sums[2];
+    get_sums<SBS, 2>(w.partial, {PS_RR, PS_RHR}, w.ps, sh, sums);
+    const double rs = sums[0], rho_ = sums[1];
+    const double rho = sc->rho2[pv], alpha = sc->alpha2[pv], omega = sc->omega2[pv];
+    int k = (omega == 0.0 || alpha == 0.0) ? -11 : sc->k2[pv] + 1;
+    if (rho == 0.0) k = -10;
+    const bool go = (rs > sc->atol2) && (k < sc->maxiter) && (k >= 0);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->k2[par] = k;
+      sc->rho2[par] = rho_;
+      sc->rs = rs;
+      sc->k = k;
+      if (!go) sc->done = 1;
+    }
+    beta = rho_ / rho * alpha / omega;
+    return go;
+  };
+  const double* __restrict__ rh = v.rh;
+  const double* __restrict__ pp = v.pp;
+  auto make_xf = [&]() {
+    const double b = beta;
+    return [=](int64_t j) { return __fma_rn(b, pp[j], rh[j]); };
+  };
+  double d = 0.0;
+  auto own = [&](int64_t r, double q) {
+    v.phat[r] = __fma_rn(beta, pp[r], rh[r]);
+    v.q[r] = q;
+    v.qh[r] = q * v.invd[r];
+    d += v.rhat[r] * q;
+  };
+  if (!spmv_sweep<TM>(M, vals, w.n, red, pre, make_xf, own)) return;
+  put_partials<SBS, 1>({d}, {PS_RQ}, w.partial, w.ps, sh);
+}
+
+// alpha, then t = A shat with shat = rh - alpha qh gathered on the fly; stores s = r - alpha q,
+// shat, t and <t,s>, <t,t>, <s,s>.
+template <typename TM>
+__global__ void __launch_bounds__(SBS) k_kst(BellView M, const TM* __restrict__ vals, KVec v, KrylovWork w, int par) {
+  __shared__ double sh[3 * (SBS / 64)];
+  __shared__ double red[(SP_PARTS - 1) * SP_GROUPS * SP_ROWS];
+  KrylovScalars* sc = w.sc;
+  if (sc->done) return;
+  double alpha = 0.0;
+  auto pre = [&]() {
+    double rq[1];
+    get_sums<SBS, 1>(w.partial, {PS_RQ}, w.ps, sh, rq);
+    alpha = sc->rho2[par] / rq[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc->alpha2[par] = alpha;
+    return true;
+  };
+  const double* __restrict__ rh = v.rh;
+  const double* __restrict__ qh = v.qh;
+  auto make_xf = [&]() {
+    const double a = alpha;
+    return [=](int64_t j) { return __fma_rn(-a, qh[j], rh[j]); };
+  };
+  double ts = 0.0, tt = 0.0, ss = 0.0;
+  auto own = [&](int64_t r, double tv) {
+    const double s = v.r[r] - alpha * v.q[r];
+    v.s[r] = s;
+    v.shat[r] = __fma_rn(-alpha, qh[r], rh[r]);
+    v.t[r] = tv;
+    ts += tv * s;
+    tt += tv * tv;
+    ss += s * s;
+  };
+  spmv_sweep<TM>(M, vals, w.n, red, pre, make_xf, own);
+  put_partials<SBS, 3>({ts, tt, ss}, {PS_TS, PS_TT, PS_SS}, w.partial, w.ps, sh);
+}
+
+// omega, x and r updates; stores rh = M r, pp = phat - omega qh and <r,r>, <rhat,r>.  The
+// first row's operands are loaded before the partial sums are reduced (one round trip).
+__global__ void __launch_bounds__(KBS) k_kxr(KVec v, KrylovWork w, int par) {
+  __shared__ double sh[3 * (KBS / 64)];
+  KrylovScalars* sc = w.sc;
+  if (sc->done) return;
+  const int64_t i0 = (int64_t)blockIdx.x * KBS + threadIdx.x;
+  double x = 0, ph = 0, shv = 0, s = 0, t = 0, qh = 0, rhat = 0, invd = 0;
+  if (i0 < w.n) {
+    x = v.x[i0]; ph = v.phat[i0]; shv = v.shat[i0]; s = v.s[i0]; t = v.t[i0];
+    qh = v.qh[i0]; rhat = v.rhat[i0]; invd = v.invd[i0];
+  }
+  double sums[3];
+  get_sums<KBS, 3>(w.partial, {PS_SS, PS_TS, PS_TT}, w.ps, sh, sums);
+  const double ss = sums[0];
+  const double omega = sums[1] / sums[2];
   const double alpha = sc->alpha2[par];
   const bool early = ss < sc->atol2;
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->omega2[par] = omega;
   double rr = 0.0, rhr = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * KBS + threadIdx.x; i < w.n; i += (int64_t)w.nb * KBS) {
+  for (int64_t i = i0; i < w.n; i += (int64_t)w.nb * KBS) {
+    if (i != i0) {
+      x = v.x[i]; ph = v.phat[i]; shv = v.shat[i]; s = v.s[i]; t = v.t[i];
+      qh = v.qh[i]; rhat = v.rhat[i]; invd = v.invd[i];
+    }
     double r;
     if (early) {
-      v.x[i] = v.x[i] + alpha * v.phat[i];
-      r = v.s[i];
+      v.x[i] = x + alpha * ph;
+      r = s;
     } else {
-      v.x[i] = v.x[i] + (alpha * v.phat[i] + omega * v.shat[i]);
-      r = v.s[i] - omega * v.t[i];
+      v.x[i] = x + (alpha * ph + omega * shv);
+      r = s - omega * t;
     }
     v.r[i] = r;
+    v.rh[i] = r * invd;
+    v.pp[i] = __fma_rn(-omega, qh, ph);
     rr += r * r;
-    rhr += v.rhat[i] * r;
+    rhr += rhat * r;
   }
-  put_partial(rr, w.partial, PS_RR, w.nb, sh);
-  __syncthreads();
-  put_partial(rhr, w.partial, PS_RHR, w.nb, sh);
+  put_partials<KBS, 2>({rr, rhr}, {PS_RR, PS_RHR}, w.partial, w.ps, sh);
 }
 
 // Reliable update of the mixed form: replace the recurrence residual by the true FP64
-// residual r = b - A64 x (and its partial sums), without restarting the Krylov space.
+// residual r = b - A64 x (with rh and the partial sums), without restarting the Krylov space.
 __global__ void __launch_bounds__(KBS) k_kreplace(BellView M, const double* __restrict__ vals, const double* __restrict__ b,
                                                  KVec v, KrylovWork w) {
-  __shared__ double sh[KBS / 64];
+  __shared__ double sh[2 * (KBS / 64)];
   if (w.sc->done) return;
   double rr = 0.0, rhr = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * KBS + threadIdx.x; i < w.n; i += (int64_t)w.nb * KBS) {
     const double r = b[i] - spmv_row<double>(M, vals, v.x, i);
     v.r[i] = r;
+    v.rh[i] = r * v.invd[i];
     rr += r * r;
     rhr += v.rhat[i] * r;
   }
-  put_partial(rr, w.partial, PS_RR, w.nb, sh);
-  __syncthreads();
-  put_partial(rhr, w.partial, PS_RHR, w.nb, sh);
+  put_partials<KBS, 2>({rr, rhr}, {PS_RR, PS_RHR}, w.partial, w.ps, sh);
 }
 
 // Generic dot product a.b finished on device into *out
@@ -371,6 +548,9 @@ void krylov_init(const BellView& M, const TM* vals, const double* b, KVec v, Kry
 }
 
 int krylov_blocks(int64_t n) { return (int)std::min<int64_t>(nblk(n), KMAXB); }
+int krylov_spmv_blocks(int64_t n) {
+  return (int)std::max<int64_t>(1, std::min<int64_t>((n + SP_ROWS * SP_GROUPS - 1) / (SP_ROWS * SP_GROUPS), SP_MAXB));
+}
 
 void krylov_replace(const BellView& M, const double* vals64, const double* b, KVec v, KrylovWork w, hipStream_t st) {
   hipLaunchKernelGGL(k_kreplace, dim3((unsigned)w.nb), dim3(KBS), 0, st, M, vals64, b, v, w);
@@ -378,19 +558,13 @@ void krylov_replace(const BellView& M, const double* vals64, const double* b, KV
 
 template <typename TM>
 void krylov_iteration(const BellView& M, const TM* vals, KVec v, KrylovWork w, int par, hipStream_t st, Prof* prof) {
-  const unsigned nb = (unsigned)w.nb;
+  const unsigned nb = (unsigned)w.nb, nbs = (unsigned)w.nbs;
   int sl;
-  sl = prof_begin(prof, PC_KVEC, st);
-  hipLaunchKernelGGL(k_kp, dim3(nb), dim3(KBS), 0, st, v, w, par);
+  sl = prof_begin(prof, PC_SPMV, st);
+  hipLaunchKernelGGL(k_kpq<TM>, dim3(nbs), dim3(SBS), 0, st, M, vals, v, w, par);
   prof_end(prof, sl, st);
   sl = prof_begin(prof, PC_SPMV, st);
-  hipLaunchKernelGGL(k_kq<TM>, dim3(nb), dim3(KBS), 0, st, M, vals, v, w);
-  prof_end(prof, sl, st);
-  sl = prof_begin(prof, PC_KVEC, st);
-  hipLaunchKernelGGL(k_ks, dim3(nb), dim3(KBS), 0, st, v, w, par);
-  prof_end(prof, sl, st);
-  sl = prof_begin(prof, PC_SPMV, st);
-  hipLaunchKernelGGL(k_kt<TM>, dim3(nb), dim3(KBS), 0, st, M, vals, v, w);
+  hipLaunchKernelGGL(k_kst<TM>, dim3(nbs), dim3(SBS), 0, st, M, vals, v, w, par);
   prof_end(prof, sl, st);
   sl = prof_begin(prof, PC_KVEC, st);
   hipLaunchKernelGGL(k_kxr, dim3(nb), dim3(KBS), 0, st, v, w, par);

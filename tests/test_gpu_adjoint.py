@@ -7,8 +7,11 @@
     singleCrystalCopper_GB.py:176-246: ad_wrapper + compute_avg_stress + update_int_vars_gp
     under autograd) for the Cu single-element objective at alpha = 1.4: gradient w.r.t. the six
     scale coefficients.
-Central differences use relative step 1e-6 (1e-4 degree for angles) with the Newton and
-Krylov tolerances tightened so the forward is smooth to ~1e-12."""
+Central differences use relative step 1e-6 (1e-4 degree for angles) with the Newton
+tolerances tightened so the forward is smooth to ~1e-12.  The Krylov tolerance is 1e-12
+relative (atol negligible): near 1e-13 Jacobi-BiCGSTAB on these tiny systems reaches the
+rounding floor of its recurrence and can break down (rhat.q = 0) in the reference semantics
+itself, whichever way the rounding falls."""
 import os
 
 import numpy as np
@@ -19,7 +22,8 @@ from conftest import GOLDEN
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
-TIGHT = {"tol": 1e-9, "rel_tol": 1e-13, "krylov_tol": 1e-13, "krylov_atol": 1e-16}
+TIGHT = {"tol": 1e-9, "rel_tol": 1e-13, "krylov_tol": 1e-12, "krylov_atol": 1e-20}
+ADJ = {"krylov_tol": 1e-12, "krylov_atol": 1e-20}
 
 
 def _poly(n_steps=3, k=2):
@@ -75,7 +79,7 @@ def test_simulate_gradient_matches_fd():
     def loss(theta, euler):
         g0, m8, R = _inputs(prob, mat, theta, euler, quat, inds)
         sig = C.simulate(prob, load, g0, m8, R, checkpoint_every=2, solver_options=TIGHT,
-                         adjoint_solver_options={"krylov_tol": 1e-13, "krylov_atol": 1e-16})
+                         adjoint_solver_options=ADJ)
         return (sig * w).sum()
 
     theta = torch.ones(9, dtype=torch.float64, device="cuda", requires_grad=True)
@@ -118,7 +122,7 @@ def test_dropin_calibration_gradient_matches_fd():
     info = [[corner, corner, bottom, top], [0, 1, 2, 2], [zero, zero, zero, lambda p: 0.0]]
     prob = C.CrystalPlasticity(mesh, dirichlet_bc_info=info, additional_info=(np.array([[1.0, 0, 0, 0]]), np.zeros(1)),
                                material=C.CALIB_COPPER.replace(tol=1e-11))
-    fwd_pred = C.ad_wrapper(prob, {"jax_solver": {}, **TIGHT}, {"krylov_tol": 1e-13, "krylov_atol": 1e-16})
+    fwd_pred = C.ad_wrapper(prob, {"jax_solver": {}, **TIGHT}, ADJ)
     idx = np.arange(0, 20, 4)
 
     def simulation(alpha):  # calibration_case1_singleCrystalCopper_GB.py:176-246

@@ -131,7 +131,10 @@ def test_assembly_parity(distort):
 
 
 def test_krylov_matches_jax_semantics():
-    """Device BiCGSTAB follows the reference iteration (same count, same x to 1e-10)."""
+    """Device BiCGSTAB follows the reference iteration: x to 1e-8 and an iteration count
+    inside the spread the oracle's own restatement shows when b is perturbed at the
+    rounding level (1e-16 relative; on this matrix that spread is several iterations, so a
+    fixed +-1 window would test luck, not the recurrence)."""
     from cpfem_amd.solver import krylov_solve, bc_x0
     from oracle import fe
     prob, oprob, u = _assembly_case(False)
@@ -152,7 +155,9 @@ def test_krylov_matches_jax_semantics():
     x0r[oprob.bc_dofs] = oprob.bc_vals
     xr, k = fe.jax_solve(A, -ref_res, x0r)
     kg = prob.solver_info[-1]["iterations"]
-    assert abs(kg - k) <= 1
+    rng = np.random.default_rng(3)
+    spread = [k] + [fe.jax_solve(A, -ref_res * (1 + 1e-16 * rng.normal(size=ref_res.shape)), x0r)[1] for _ in range(24)]
+    assert min(spread) - 1 <= kg <= max(spread) + 1, (kg, sorted(spread))
     assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-8
     # mixed precision reaches the same threshold
     xm = krylov_solve(prob, mat, -res, x0, dict(precision="mixed")).cpu().numpy()
