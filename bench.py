@@ -94,6 +94,50 @@ def cpu_baseline(prob, w, params, sol, per_step, sample_cells=512, krylov_iters=
                         f"est. {step_s:.1f} s per load step"))
 
 
+def adjoint_bench(n=None, steps=None, k=10):
+    """Gradient of a calibration loss w.r.t. five hardening scales through the whole load
+    history (hand-written adjoint, history checkpointed every k steps).  Workload: the
+    24^3 / 64-grain calibration case, loss = sum over steps of (szz - target)^2 with the
+    target from the hidden parameters and the query = hidden x (1 + U(-0.2, 0.2)) (seeded)."""
+    import numpy as np
+    import torch
+    import cpfem_amd as C
+    from cpfem_amd import configs
+    w = configs.config4(**({} if n is None else {"n": n}))
+    if steps is not None:
+        w.disps, w.ts = w.disps[:steps + 1], w.ts[:steps + 1]
+    prob = configs.build_problem(w)
+    nc, Ns, m = len(w.mesh.cells), prob.num_slip_sys, w.material
+    load = C.LoadHistory(w.disps, w.ts, w.dirichlet_bc_info)
+    R = torch.as_tensor(C.get_rot_mat(w.quat)[w.cell_ori_inds], device="cuda")[:, None].expand(nc, 8, 3, 3)
+
+    def inputs(theta):  # theta: scales of (g0, h0, g_sat, a, m)
+        base = torch.tensor([m.xm, m.gss_a, m.h, m.t_sat, m.r, m.C11, m.C12, m.C44], dtype=torch.float64, device="cuda")
+        one = torch.ones((), dtype=torch.float64, device="cuda")
+        sc = torch.stack([theta[4], theta[3], theta[1], theta[2], one, one, one, one])
+        return (m.g0 * theta[0]).expand(nc, 8, Ns), (base * sc).expand(nc, 8, 8)
+
+    hidden = torch.ones(5, dtype=torch.float64, device="cuda")
+    rng = np.random.default_rng(0)
+    query = torch.as_tensor(1.0 + rng.uniform(-0.2, 0.2, size=5), device="cuda").requires_grad_(True)
+    with torch.no_grad():
+        g0, m8 = inputs(hidden)
+        target = C.simulate(prob, load, g0, m8, R, checkpoint_every=k)[:, 2, 2]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g0, m8 = inputs(query)
+    szz = C.simulate(prob, load, g0, m8, R, checkpoint_every=k)[:, 2, 2]
+    loss = ((szz - target) ** 2).sum()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    return dict(workload=w.name, load_steps=load.n_steps, checkpoint_every=k, n_qp=w.n_qp,
+                forward_s=t1 - t0, backward_s=t2 - t1, grad_s=t2 - t0, loss=float(loss),
+                grad=[float(v) for v in query.grad.cpu()])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +150,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--compare-mixed", action="store_true", help="rerun the timed steps in the other precision")
     ap.add_argument("--copy-calibration", action="store_true", help="also run a plain copy kernel (counter scale)")
+    ap.add_argument("--adjoint", action="store_true", help="also time the history gradient of the calibration workload")
+    ap.add_argument("--adjoint-steps", type=int, default=None, help="load steps of the adjoint workload (default: all 50)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it", "n_gpus": args.gpus}))
@@ -214,6 +260,11 @@ def main():
         N.check(L.cpfem_profile(h, 0, 1))
         cms = ms[7] / ti[7]
         detail["copy_calibration"] = dict(bytes=16 * n, avg_ms=cms, GBps=16 * n / (cms * 1e-3) / 1e9)
+
+    if args.adjoint:
+        log("[bench] adjoint: calibration-workload history gradient ...")
+        detail["adjoint"] = adjoint_bench(steps=args.adjoint_steps)
+        log(f"[bench] adjoint: forward {detail['adjoint']['forward_s']:.2f}s, backward {detail['adjoint']['backward_s']:.2f}s")
 
     cpu = None
     if not args.no_cpu:

@@ -12,7 +12,9 @@ from .models import CrystalPlasticity
 from .problem import BlockEllMatrix, Problem
 from .solver import solver
 from .adjoint import ad_wrapper, implicit_vjp, simulate, LoadHistory
+from .rotations import alpha_rotate_tensor_rank_2, get_rot_mat, rotate_theta_transfer
 
 __all__ = ["CrystalPlasticity", "Problem", "solver", "Mesh", "box_mesh", "get_meshio_cell_type", "Material",
            "BlockEllMatrix", "COPPER", "TANTALUM", "STEEL304", "CALIB_COPPER", "CALIB_TANTALUM",
-           "CALIB_STEEL304", "SENS_ROT", "PRESETS", "ad_wrapper", "implicit_vjp", "simulate", "LoadHistory"]
+           "CALIB_STEEL304", "SENS_ROT", "PRESETS", "ad_wrapper", "implicit_vjp", "simulate", "LoadHistory",
+           "get_rot_mat", "rotate_theta_transfer", "alpha_rotate_tensor_rank_2"]

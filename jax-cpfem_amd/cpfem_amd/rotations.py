@@ -1,4 +1,8 @@
-"""Orientation helpers: quaternion and Euler-angle rotations, uniform SO(3) sampling."""
+"""Orientation helpers: quaternion and Euler-angle rotations, uniform SO(3) sampling.
+
+``rotate_theta_transfer`` and ``alpha_rotate_tensor_rank_2`` accept torch tensors and then
+stay differentiable under autograd (the orientation-design path differentiates the loss
+with respect to the Euler angles through them)."""
 import numpy as np
 
 
@@ -38,3 +42,30 @@ def uniform_quaternions(n, rng):
     z = b * np.sin(2 * np.pi * u3)
     w = b * np.cos(2 * np.pi * u3)
     return np.stack([w, x, y, z], axis=1)
+
+
+def rotate_theta_transfer(angles_deg):
+    """(..., 3) extrinsic z-y-x Euler angles in degrees -> (..., 3, 3) rotation matrices
+    (poly_sensitivity_rot_mesh2.py:41-53, scipy ``from_euler('zyx', degrees=True)``).
+    Torch input gives a differentiable torch result; anything else goes through NumPy."""
+    try:
+        import torch
+    except ImportError:  # pragma: no cover
+        torch = None
+    if torch is None or not isinstance(angles_deg, torch.Tensor):
+        return euler_zyx(angles_deg)
+    a = torch.deg2rad(angles_deg)
+    ca, sa = torch.cos(a[..., 0]), torch.sin(a[..., 0])
+    cb, sb = torch.cos(a[..., 1]), torch.sin(a[..., 1])
+    cc, sc = torch.cos(a[..., 2]), torch.sin(a[..., 2])
+    z, o = torch.zeros_like(ca), torch.ones_like(ca)
+    Rz = torch.stack([torch.stack([ca, -sa, z], -1), torch.stack([sa, ca, z], -1), torch.stack([z, z, o], -1)], -2)
+    Ry = torch.stack([torch.stack([cb, z, sb], -1), torch.stack([z, o, z], -1), torch.stack([-sb, z, cb], -1)], -2)
+    Rx = torch.stack([torch.stack([o, z, z], -1), torch.stack([z, cc, -sc], -1), torch.stack([z, sc, cc], -1)], -2)
+    return Rx @ Ry @ Rz
+
+
+def alpha_rotate_tensor_rank_2(R_a, R_b):
+    """R_a @ R_b per point: a newly applied rotation composed with the existing
+    orientation (poly_sensitivity_rot_mesh2.py:60-65, applied to params[3] at :203)."""
+    return R_a @ R_b

@@ -49,7 +49,7 @@ def _poly(n_steps=3, k=2):
 
 def _inputs(prob, mat, theta, euler, quat, inds):
     """Per-point (g0, mat8, R) from 9 scales and per-grain Euler angles (differentiable)."""
-    from cpfem_amd.rotations import get_rot_mat
+    from cpfem_amd.rotations import alpha_rotate_tensor_rank_2, get_rot_mat, rotate_theta_transfer
     nc = prob.fes[0].num_cells
     base = torch.tensor([mat.xm, mat.gss_a, mat.h, mat.t_sat, mat.r, mat.C11, mat.C12, mat.C44],
                         dtype=torch.float64, device="cuda")
@@ -57,17 +57,10 @@ def _inputs(prob, mat, theta, euler, quat, inds):
     scale = torch.stack([theta[4], theta[3], theta[1], theta[2], theta[5], theta[6], theta[7], theta[8]])
     mat8 = (base * scale).expand(nc, 8, 8)
     g0 = (mat.g0 * theta[0]).expand(nc, 8, 12)
-    a = torch.deg2rad(euler)
-    ca, sa = torch.cos(a[:, 0]), torch.sin(a[:, 0])
-    cb, sb = torch.cos(a[:, 1]), torch.sin(a[:, 1])
-    cc, sc = torch.cos(a[:, 2]), torch.sin(a[:, 2])
-    z, o = torch.zeros_like(ca), torch.ones_like(ca)
-    Rz = torch.stack([torch.stack([ca, -sa, z], -1), torch.stack([sa, ca, z], -1), torch.stack([z, z, o], -1)], -2)
-    Ry = torch.stack([torch.stack([cb, z, sb], -1), torch.stack([z, o, z], -1), torch.stack([-sb, z, cb], -1)], -2)
-    Rx = torch.stack([torch.stack([o, z, z], -1), torch.stack([z, cc, -sc], -1), torch.stack([z, sc, cc], -1)], -2)
-    Re = Rx @ Ry @ Rz
+    # poly_sensitivity_rot_mesh2.py:41-65,203: R = R_euler(theta) @ R_quat per grain
+    Re = rotate_theta_transfer(euler)
     Rq = torch.as_tensor(get_rot_mat(quat), device="cuda")
-    R = (Re @ Rq)[torch.as_tensor(inds, device="cuda")]
+    R = alpha_rotate_tensor_rank_2(Re, Rq)[torch.as_tensor(inds, device="cuda")]
     return g0, mat8, R.unsqueeze(1).expand(nc, 8, 3, 3)
 
 
