@@ -14,11 +14,11 @@
 //     k = -11 on alpha/omega = 0, -10 on rho_ = 0, else k + 1
 //
 // Design: three kernels per iteration.  p is never stored: with rh = M r, qh = M q and
-// pp = M (p - omega q) kept per row (M is diagonal), phat = rh + beta pp and
-// shat = rh - alpha qh are formed on the fly wherever the SpMV gathers them, so the p and s
-// updates ride inside the two SpMV kernels and only the x/r update is a kernel of its own.
-// (Same recurrence as JAX's in exact arithmetic; the Jacobi scaling is applied before instead
-// of after the vector sums, which moves results by rounding only.)  Every dot product is
+// pp = M (p - omega q) kept per row (M is diagonal), q = A phat = A rh + beta A pp and
+// t = A shat = A rh - alpha A qh, so the p and s updates ride inside the two SpMV kernels and
+// only the x/r update is a kernel of its own.  (Same recurrence as JAX's in exact
+// arithmetic; the Jacobi scaling and the SpMV are applied before instead of after the vector
+// sums, which moves results by rounding only.)  Every dot product is
 // fused into the kernel producing its operand as per-block partial sums, which the consuming
 // kernel adds in a fixed order (bitwise reproducible run to run, no atomics).  All scalars
 // stay on the device; each kernel returns at once when the device-side `done` flag is set,
@@ -254,13 +254,17 @@ __global__ void __launch_bounds__(KBS) k_kinit2(KrylovWork w) {
 // wave of part p takes slots p, p+4, ...), so a launch keeps four times as many waves in
 // flight as a thread-per-row SpMV while every wave still reads the column-major values
 // coalesced.  The four partial row sums meet in LDS and the part-0 wave adds them in fixed
-// order (bitwise reproducible).  The gathered operand is a functor of the column index, so
-// the p and s updates of the iteration are folded into the gather.
+// order (bitwise reproducible).
 //
-// Latency: the column indices and block values do not depend on the iteration's scalars, so
-// on 27-slot meshes a block issues them for its first row group before it reduces the
-// partial sums the scalars come from; the matrix stream and the reduction then share one
-// memory round trip instead of following each other.
+// Latency: the iteration's operand is a combination u + c v of two stored vectors whose
+// coefficient c (beta or -alpha) comes from a global reduction the same kernel finishes.
+// The SpMV therefore accumulates A u and A v separately and combines them afterwards
+// (A (u + c v) = A u + c A v; equal up to rounding), so the matrix stream never waits for the
+// reduction: a kernel loads its scalars and the first row's epilogue operands, issues its
+// column indices before it tests the device-side `done` flag, streams values and gathers,
+// and meanwhile loads the partial sums it reduces.  (Measured on the 32^3 matrix: this form
+// at 1024-thread blocks and 4 waves/SIMD beats both a single-accumulator form that reduces
+// first and smaller blocks at higher occupancy.)
 constexpr int SP_ROWS = 64;
 constexpr int SP_PARTS = 4;
 constexpr int SP_GROUPS = 4;
@@ -268,18 +272,17 @@ constexpr int SP_BLOCK_ROWS = SP_ROWS * SP_GROUPS;
 constexpr int SBS = SP_ROWS * SP_PARTS * SP_GROUPS;
 constexpr int SP_MAXB = KMAXB;
 constexpr int SP_NS = (27 + SP_PARTS - 1) / SP_PARTS;  // slots per part on 27-slot meshes
+constexpr int SP_RED = (SP_PARTS - 1) * SP_GROUPS * SP_ROWS;
 
 struct SpmvLane {
   int lane, part, group;
   __device__ SpmvLane() : lane(threadIdx.x & 63), part((threadIdx.x >> 6) / SP_GROUPS), group((threadIdx.x >> 6) % SP_GROUPS) {}
 };
 
-// One part's share of one row on a 27-slot mesh: column nodes and 3x3-block row values.
-template <typename TM>
-struct SlotShare {
+// Column nodes of one part's share of a row on a 27-slot mesh.
+struct SlotCols {
   int32_t j[SP_NS];
-  TM a[SP_NS][3];
-  __device__ __forceinline__ void load_cols(const BellView& M, int64_t r, int part) {
+  __device__ __forceinline__ void load(const BellView& M, int64_t r, int part) {
     const int64_t i = r / 3;
 #pragma unroll
     for (int k = 0; k < SP_NS; ++k) {
@@ -287,194 +290,276 @@ struct SlotShare {
       j[k] = s < 27 ? M.col[(int64_t)s * M.n_nodes + i] : 0;
     }
   }
-  __device__ __forceinline__ void load_vals(const BellView& M, const TM* __restrict__ vals, int64_t r, int part) {
-    const int64_t nrow = 3 * M.n_nodes;
-#pragma unroll
-    for (int k = 0; k < SP_NS; ++k) {
-      const int s = part + SP_PARTS * k;
-      const TM* v = vals + (3 * (int64_t)s) * nrow + r;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) a[k][c] = s < 27 ? v[c * nrow] : TM(0);
-    }
-  }
-  template <typename XF>
-  __device__ __forceinline__ double dot(XF xf, int part) const {
-    double acc = 0.0;
-#pragma unroll
-    for (int k = 0; k < SP_NS; ++k) {
-      if (part + SP_PARTS * k < 27) {
-        const int64_t jj = 3 * (int64_t)j[k];
-        acc += (double)a[k][0] * xf(jj) + (double)a[k][1] * xf(jj + 1) + (double)a[k][2] * xf(jj + 2);
-      }
-    }
-    return acc;
-  }
 };
 
-// Any slot count (meshes that are not structured hex boxes).
-template <typename TM, typename XF>
-__device__ __forceinline__ double spmv_part_any(const BellView& M, const TM* __restrict__ vals, XF xf, int64_t r, int part) {
+// a1 += A[r, part's slots] u, a2 += A[r, part's slots] v.
+template <typename TM>
+__device__ __forceinline__ void spmv2_27(const BellView& M, const TM* __restrict__ vals, const double* __restrict__ u,
+                                         const double* __restrict__ v, const SlotCols& jc, int64_t r, int part, double& a1,
+                                         double& a2) {
+  const int64_t nrow = 3 * M.n_nodes;
+#pragma unroll
+  for (int k = 0; k < SP_NS; ++k) {
+    const int s = part + SP_PARTS * k;
+    if (s < 27) {
+      const TM* a = vals + (3 * (int64_t)s) * nrow + r;
+      const int64_t jj = 3 * (int64_t)jc.j[k];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double m = (double)a[c * nrow];
+        a1 += m * u[jj + c];
+        a2 += m * v[jj + c];
+      }
+    }
+  }
+}
+
+template <typename TM>
+__device__ __forceinline__ void spmv2_any(const BellView& M, const TM* __restrict__ vals, const double* __restrict__ u,
+                                          const double* __restrict__ v, int64_t r, int part, double& a1, double& a2) {
   const int64_t nrow = 3 * M.n_nodes;
   const int64_t i = r / 3;
   const int ns = M.nslot[i];
-  double acc = 0.0;
   for (int s = part; s < ns; s += SP_PARTS) {
-    const int64_t j = M.col[(int64_t)s * M.n_nodes + i];
-    const TM* v = vals + (3 * (int64_t)s) * nrow + r;
-    acc += (double)v[0] * xf(3 * j) + (double)v[nrow] * xf(3 * j + 1) + (double)v[2 * nrow] * xf(3 * j + 2);
+    const int64_t jj = 3 * (int64_t)M.col[(int64_t)s * M.n_nodes + i];
+    const TM* a = vals + (3 * (int64_t)s) * nrow + r;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double m = (double)a[c * nrow];
+      a1 += m * u[jj + c];
+      a2 += m * v[jj + c];
+    }
   }
-  return acc;
 }
 
-// Adds the parts' row sums: complete in the part-0 waves.  red: (SP_PARTS - 1) * SBS / SP_PARTS doubles.
-__device__ __forceinline__ double row_sum(double acc, const SpmvLane& t, double* red) {
-  if (t.part) red[((t.part - 1) * SP_GROUPS + t.group) * SP_ROWS + t.lane] = acc;
-  __syncthreads();
+// Part sums of (a1, a2) into the part-0 waves; red: 2 * SP_RED doubles.  Leaves a barrier
+// pending: the caller's next __syncthreads() must precede any reuse of red.
+__device__ __forceinline__ void part_sums_write(double a1, double a2, const SpmvLane& t, double* red) {
+  if (t.part) {
+    const int o = ((t.part - 1) * SP_GROUPS + t.group) * SP_ROWS + t.lane;
+    red[o] = a1;
+    red[SP_RED + o] = a2;
+  }
+}
+__device__ __forceinline__ void part_sums_read(double& a1, double& a2, const SpmvLane& t, const double* red) {
   if (t.part == 0) {
 #pragma unroll
-    for (int p = 1; p < SP_PARTS; ++p) acc += red[((p - 1) * SP_GROUPS + t.group) * SP_ROWS + t.lane];
+    for (int p = 1; p < SP_PARTS; ++p) {
+      const int o = ((p - 1) * SP_GROUPS + t.group) * SP_ROWS + t.lane;
+      a1 += red[o];
+      a2 += red[SP_RED + o];
+    }
   }
-  __syncthreads();
-  return acc;
 }
 
-// Row-group loop of the two SpMV kernels.  pre(): scalars (called once, after the first
-// group's matrix loads are in flight; returns false to stop, uniformly over the grid).
-// make_xf(): the gathered operand.  own(r, y): row epilogue in the part-0 waves.  Returns
-// false if pre() stopped the sweep.
-template <typename TM, typename Pre, typename MakeXF, typename Own>
-__device__ __forceinline__ bool spmv_sweep(const BellView& M, const TM* __restrict__ vals, int64_t n, double* red, Pre pre,
-                                           MakeXF make_xf, Own own) {
-  const SpmvLane t;
-  const bool s27 = M.n_slots == 27;
-  int64_t r0 = (int64_t)blockIdx.x * SP_BLOCK_ROWS;
-  int64_t r = r0 + t.group * SP_ROWS + t.lane;
-  SlotShare<TM> sh27;
-  if (s27 && r < n) {
-    sh27.load_cols(M, r, t.part);
-    sh27.load_vals(M, vals, r, t.part);
-  }
-  if (!pre()) return false;
-  const auto xf = make_xf();
-  for (; r0 < n; r0 += (int64_t)gridDim.x * SP_BLOCK_ROWS) {
-    r = r0 + t.group * SP_ROWS + t.lane;
-    double acc = 0.0;
-    if (r < n) {
-      if (s27) {
-        if (r0 != (int64_t)blockIdx.x * SP_BLOCK_ROWS) {
-          sh27.load_cols(M, r, t.part);
-          sh27.load_vals(M, vals, r, t.part);
-        }
-        acc = sh27.dot(xf, t.part);
-      } else {
-        acc = spmv_part_any<TM>(M, vals, xf, r, t.part);
-      }
+// One row group: (A u, A v) for rows r0 + 64 group + lane, complete in the part-0 waves.
+template <typename TM>
+__device__ __forceinline__ void spmv2_group(const BellView& M, const TM* __restrict__ vals, const double* __restrict__ u,
+                                            const double* __restrict__ v, int64_t r, int64_t n, const SpmvLane& t, double* red,
+                                            double& a1, double& a2) {
+  a1 = a2 = 0.0;
+  if (r < n) {
+    if (M.n_slots == 27) {
+      SlotCols jc;
+      jc.load(M, r, t.part);
+      spmv2_27<TM>(M, vals, u, v, jc, r, t.part, a1, a2);
+    } else {
+      spmv2_any<TM>(M, vals, u, v, r, t.part, a1, a2);
     }
-    acc = row_sum(acc, t, red);
-    if (t.part == 0 && r < n) own(r, acc);
+  }
+  part_sums_write(a1, a2, t, red);
+  __syncthreads();
+  part_sums_read(a1, a2, t, red);
+  __syncthreads();
+}
+
+// Sums of NSL partial slots from registers already loaded (p[j]: this thread's share), with
+// the part sums of the first row group riding on the same barrier.
+template <int NSL>
+__device__ __forceinline__ void finish_first(double (&p)[NSL], double& a1, double& a2, const SpmvLane& t, double* sh,
+                                             double* red) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < NSL; ++j) {
+    const double s = wave_sum(p[j]);
+    if (lane == 0) sh[j * (SBS / 64) + wid] = s;
+  }
+  part_sums_write(a1, a2, t, red);
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < NSL; ++j) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < SBS / 64; ++i) s += sh[j * (SBS / 64) + i];
+    p[j] = s;
+  }
+  part_sums_read(a1, a2, t, red);
+  __syncthreads();
+}
+
+// First row group of an SpMV kernel: column indices issued before the done test.  Returns
+// false if the solve is finished (uniform over the grid).
+template <typename TM>
+__device__ __forceinline__ bool first_group(const BellView& M, const TM* __restrict__ vals, const double* __restrict__ u,
+                                            const double* __restrict__ v, int64_t r, int64_t n, const SpmvLane& t, int done,
+                                            double& a1, double& a2) {
+  a1 = a2 = 0.0;
+  const bool s27 = M.n_slots == 27;
+  SlotCols jc;
+  if (s27 && r < n) jc.load(M, r, t.part);
+  if (done) return false;
+  if (r < n) {
+    if (s27)
+      spmv2_27<TM>(M, vals, u, v, jc, r, t.part, a1, a2);
+    else
+      spmv2_any<TM>(M, vals, u, v, r, t.part, a1, a2);
   }
   return true;
 }
 
-// Start of iteration j: convergence test of iteration j-1, beta, then
-// q = A phat with phat = rh + beta pp gathered on the fly; stores phat, q, qh and <rhat,q>.
+// Start of iteration j: q = A phat with phat = rh + beta pp, where beta needs <rhat,r> of
+// iteration j-1; also the convergence test of iteration j-1.  Stores phat, q, qh = M q and
+// the partials of <rhat,q>.
 template <typename TM>
 __global__ void __launch_bounds__(SBS) k_kpq(BellView M, const TM* __restrict__ vals, KVec v, KrylovWork w, int par) {
   __shared__ double sh[2 * (SBS / 64)];
-  __shared__ double red[(SP_PARTS - 1) * SP_GROUPS * SP_ROWS];
+  __shared__ double red[2 * SP_RED];
   KrylovScalars* sc = w.sc;
-  if (sc->done) return;
-  double beta = 0.0;
-  auto pre = [&]() {
-    const int pv = par ^ 1;
-    double sums[2];
-    get_sums<SBS, 2>(w.partial, {PS_RR, PS_RHR}, w.ps, sh, sums);
-    const double rs = sums[0], rho_ = sums[1];
-    const double rho = sc->rho2[pv], alpha = sc->alpha2[pv], omega = sc->omega2[pv];
-    int k = (omega == 0.0 || alpha == 0.0) ? -11 : sc->k2[pv] + 1;
-    if (rho == 0.0) k = -10;
-    const bool go = (rs > sc->atol2) && (k < sc->maxiter) && (k >= 0);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      sc->k2[par] = k;
-      sc->rho2[par] = rho_;
-      sc->rs = rs;
-      sc->k = k;
-      if (!go) sc->done = 1;
-    }
-    beta = rho_ / rho * alpha / omega;
-    return go;
-  };
+  const int pv = par ^ 1;
+  const int done = sc->done;
+  const double rho = sc->rho2[pv], alpha = sc->alpha2[pv], omega = sc->omega2[pv], atol2 = sc->atol2;
+  const int kprev = sc->k2[pv], maxiter = sc->maxiter;
   const double* __restrict__ rh = v.rh;
   const double* __restrict__ pp = v.pp;
-  auto make_xf = [&]() {
-    const double b = beta;
-    return [=](int64_t j) { return __fma_rn(b, pp[j], rh[j]); };
-  };
+  const SpmvLane t;
+  const int64_t n = w.n, step = (int64_t)gridDim.x * SP_BLOCK_ROWS;
+  int64_t r = (int64_t)blockIdx.x * SP_BLOCK_ROWS + t.group * SP_ROWS + t.lane;
+  double e0 = 0.0, e1 = 0.0, e2 = 0.0, e3 = 0.0;  // rhat, M, rh, pp of the first row
+  if (t.part == 0 && r < n) {
+    e0 = v.rhat[r];
+    e1 = v.invd[r];
+    e2 = rh[r];
+    e3 = pp[r];
+  }
+  double a1, a2;
+  if (!first_group<TM>(M, vals, rh, pp, r, n, t, done, a1, a2)) return;
+  double p[2] = {0.0, 0.0};
+  for (int i = threadIdx.x; i < w.ps; i += SBS) {
+    p[0] += w.partial[PS_RR * w.ps + i];
+    p[1] += w.partial[PS_RHR * w.ps + i];
+  }
+  finish_first<2>(p, a1, a2, t, sh, red);
+  const double rs = p[0], rho_ = p[1];
+  int k = (omega == 0.0 || alpha == 0.0) ? -11 : kprev + 1;
+  if (rho == 0.0) k = -10;
+  const bool go = (rs > atol2) && (k < maxiter) && (k >= 0);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->k2[par] = k;
+    sc->rho2[par] = rho_;
+    sc->rs = rs;
+    sc->k = k;
+    if (!go) sc->done = 1;
+  }
+  if (!go) return;
+  const double beta = rho_ / rho * alpha / omega;
   double d = 0.0;
-  auto own = [&](int64_t r, double q) {
-    v.phat[r] = __fma_rn(beta, pp[r], rh[r]);
-    v.q[r] = q;
-    v.qh[r] = q * v.invd[r];
-    d += v.rhat[r] * q;
-  };
-  if (!spmv_sweep<TM>(M, vals, w.n, red, pre, make_xf, own)) return;
+  bool first = true;
+  for (;;) {
+    if (t.part == 0 && r < n) {
+      if (!first) {
+        e0 = v.rhat[r];
+        e1 = v.invd[r];
+        e2 = rh[r];
+        e3 = pp[r];
+      }
+      const double q = __fma_rn(beta, a2, a1);
+      v.phat[r] = __fma_rn(beta, e3, e2);
+      v.q[r] = q;
+      v.qh[r] = q * e1;
+      d += e0 * q;
+    }
+    first = false;
+    r += step;
+    if (r - t.group * SP_ROWS - t.lane >= n) break;  // uniform over the block
+    spmv2_group<TM>(M, vals, rh, pp, r, n, t, red, a1, a2);
+  }
   put_partials<SBS, 1>({d}, {PS_RQ}, w.partial, w.ps, sh);
 }
 
-// alpha, then t = A shat with shat = rh - alpha qh gathered on the fly; stores s = r - alpha q,
-// shat, t and <t,s>, <t,t>, <s,s>.
+// t = A shat with shat = rh - alpha qh, alpha = rho / <rhat,q>.  Stores s = r - alpha q, shat,
+// t and the partials of <t,s>, <t,t>, <s,s>.
 template <typename TM>
 __global__ void __launch_bounds__(SBS) k_kst(BellView M, const TM* __restrict__ vals, KVec v, KrylovWork w, int par) {
   __shared__ double sh[3 * (SBS / 64)];
-  __shared__ double red[(SP_PARTS - 1) * SP_GROUPS * SP_ROWS];
+  __shared__ double red[2 * SP_RED];
   KrylovScalars* sc = w.sc;
-  if (sc->done) return;
-  double alpha = 0.0;
-  auto pre = [&]() {
-    double rq[1];
-    get_sums<SBS, 1>(w.partial, {PS_RQ}, w.ps, sh, rq);
-    alpha = sc->rho2[par] / rq[0];
-    if (blockIdx.x == 0 && threadIdx.x == 0) sc->alpha2[par] = alpha;
-    return true;
-  };
+  const int done = sc->done;
+  const double rho = sc->rho2[par];
   const double* __restrict__ rh = v.rh;
   const double* __restrict__ qh = v.qh;
-  auto make_xf = [&]() {
-    const double a = alpha;
-    return [=](int64_t j) { return __fma_rn(-a, qh[j], rh[j]); };
-  };
+  const SpmvLane t;
+  const int64_t n = w.n, step = (int64_t)gridDim.x * SP_BLOCK_ROWS;
+  int64_t r = (int64_t)blockIdx.x * SP_BLOCK_ROWS + t.group * SP_ROWS + t.lane;
+  double e0 = 0.0, e1 = 0.0, e2 = 0.0, e3 = 0.0;  // r, q, rh, qh of the first row
+  if (t.part == 0 && r < n) {
+    e0 = v.r[r];
+    e1 = v.q[r];
+    e2 = rh[r];
+    e3 = qh[r];
+  }
+  double a1, a2;
+  if (!first_group<TM>(M, vals, rh, qh, r, n, t, done, a1, a2)) return;
+  double p[1] = {0.0};
+  for (int i = threadIdx.x; i < w.ps; i += SBS) p[0] += w.partial[PS_RQ * w.ps + i];
+  finish_first<1>(p, a1, a2, t, sh, red);
+  const double alpha = rho / p[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->alpha2[par] = alpha;
   double ts = 0.0, tt = 0.0, ss = 0.0;
-  auto own = [&](int64_t r, double tv) {
-    const double s = v.r[r] - alpha * v.q[r];
-    v.s[r] = s;
-    v.shat[r] = __fma_rn(-alpha, qh[r], rh[r]);
-    v.t[r] = tv;
-    ts += tv * s;
-    tt += tv * tv;
-    ss += s * s;
-  };
-  spmv_sweep<TM>(M, vals, w.n, red, pre, make_xf, own);
+  bool first = true;
+  for (;;) {
+    if (t.part == 0 && r < n) {
+      if (!first) {
+        e0 = v.r[r];
+        e1 = v.q[r];
+        e2 = rh[r];
+        e3 = qh[r];
+      }
+      const double tv = __fma_rn(-alpha, a2, a1);
+      const double s = e0 - alpha * e1;
+      v.s[r] = s;
+      v.shat[r] = __fma_rn(-alpha, e3, e2);
+      v.t[r] = tv;
+      ts += tv * s;
+      tt += tv * tv;
+      ss += s * s;
+    }
+    first = false;
+    r += step;
+    if (r - t.group * SP_ROWS - t.lane >= n) break;
+    spmv2_group<TM>(M, vals, rh, qh, r, n, t, red, a1, a2);
+  }
   put_partials<SBS, 3>({ts, tt, ss}, {PS_TS, PS_TT, PS_SS}, w.partial, w.ps, sh);
 }
 
-// omega, x and r updates; stores rh = M r, pp = phat - omega qh and <r,r>, <rhat,r>.  The
-// first row's operands are loaded before the partial sums are reduced (one round trip).
+// omega, x and r updates; stores rh = M r, pp = phat - omega qh and the partials of <r,r>,
+// <rhat,r>.  Scalars, the first row's operands and the partials are loaded together.
 __global__ void __launch_bounds__(KBS) k_kxr(KVec v, KrylovWork w, int par) {
   __shared__ double sh[3 * (KBS / 64)];
   KrylovScalars* sc = w.sc;
-  if (sc->done) return;
+  const int done = sc->done;
+  const double alpha = sc->alpha2[par], atol2 = sc->atol2;
   const int64_t i0 = (int64_t)blockIdx.x * KBS + threadIdx.x;
   double x = 0, ph = 0, shv = 0, s = 0, t = 0, qh = 0, rhat = 0, invd = 0;
-  if (i0 < w.n) {
+  if (i0 < w.n) {  // issued before the done test (one round trip for both)
     x = v.x[i0]; ph = v.phat[i0]; shv = v.shat[i0]; s = v.s[i0]; t = v.t[i0];
     qh = v.qh[i0]; rhat = v.rhat[i0]; invd = v.invd[i0];
   }
+  if (done) return;
   double sums[3];
   get_sums<KBS, 3>(w.partial, {PS_SS, PS_TS, PS_TT}, w.ps, sh, sums);
   const double ss = sums[0];
   const double omega = sums[1] / sums[2];
-  const double alpha = sc->alpha2[par];
-  const bool early = ss < sc->atol2;
+  const bool early = ss < atol2;
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->omega2[par] = omega;
   double rr = 0.0, rhr = 0.0;
   for (int64_t i = i0; i < w.n; i += (int64_t)w.nb * KBS) {
