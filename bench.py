@@ -91,8 +91,11 @@ def cpu_baseline(prob, w, params, sol, per_step, sample_cells=512, krylov_iters=
         t0 = time.perf_counter()
         cp_local.local_update(ug, Fp, g, sl, R, om, dt, max_sub_step=mat.max_sub_step, want_tangent=False)
         t_post = (time.perf_counter() - t0) / (S * 8)
+        prob.set_params(params)  # the snapshot history that belongs to `sol`
         _, mat_vals = prob.assemble(sol[0].reshape(-1))
         Asp = __import__("cpfem_amd").BlockEllMatrix(prob.ctx, mat_vals).to_scipy()
+        if not np.isfinite(Asp.data).all():
+            raise RuntimeError("CPU baseline: the sampled Jacobian is not finite")
         # seeded right-hand side far from convergence, so that all krylov_iters iterations run
         b = Asp @ np.random.default_rng(0).normal(size=Asp.shape[0])
         inv_d = 1.0 / Asp.diagonal()
@@ -139,13 +142,14 @@ def adjoint_bench(n=None, steps=None, k=10):
     hidden = torch.ones(5, dtype=torch.float64, device="cuda")
     rng = np.random.default_rng(0)
     query = torch.as_tensor(1.0 + rng.uniform(-0.2, 0.2, size=5), device="cuda").requires_grad_(True)
+    opts = dict(w.solver_options)
     with torch.no_grad():
         g0, m8 = inputs(hidden)
-        target = C.simulate(prob, load, g0, m8, R, checkpoint_every=k)[:, 2, 2]
+        target = C.simulate(prob, load, g0, m8, R, checkpoint_every=k, solver_options=opts)[:, 2, 2]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     g0, m8 = inputs(query)
-    szz = C.simulate(prob, load, g0, m8, R, checkpoint_every=k)[:, 2, 2]
+    szz = C.simulate(prob, load, g0, m8, R, checkpoint_every=k, solver_options=opts)[:, 2, 2]
     loss = ((szz - target) ** 2).sum()
     torch.cuda.synchronize()
     t1 = time.perf_counter()

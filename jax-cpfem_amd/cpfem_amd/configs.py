@@ -29,6 +29,7 @@ class Workload:
     phase_inds: np.ndarray = None
     phase_materials: tuple = None
     precision: str = "fp64"
+    solver_options: dict = dataclasses.field(default_factory=dict)  # the reference driver's options
 
     @property
     def n_qp(self):
@@ -106,7 +107,9 @@ def config4(seed=0, n=24):
     mat = M.Material(slip="fcc12", g0=90.0, h=450.0, t_sat=280.0, gss_a=2.2, xm=0.02, r=1.4,
                      C11=204.6e3, C12=137.7e3, C44=126.2e3, layout="9")
     d, t = _history(0.05, 50)
-    return Workload(f"C4 calibration {n}^3, 64 grains", mesh, mat, quat, ids, d, t)
+    # solver options of the steel calibration driver (calibration_case4_UQ_polyCrystalSteel_1D_GB.py:77)
+    return Workload(f"C4 calibration {n}^3, 64 grains", mesh, mat, quat, ids, d, t,
+                    solver_options={"tol": 1e-7, "line_search_flag": True})
 
 
 def config5(seed=0, n=64):
@@ -122,8 +125,10 @@ def config5(seed=0, n=64):
     grain_phase = np.zeros(300, dtype=np.int64)
     grain_phase[hard_grains] = 1
     d, t = _history(0.02, 20)
+    # solver options of the dual-phase driver (polycrystal_DPsteel_inhomo.py:227)
     return Workload(f"C5 dual-phase BCC {n}^3, 300 grains", mesh, soft, quat, ids, d, t,
-                    phase_inds=grain_phase[ids], phase_materials=(soft, hard), precision="mixed")
+                    phase_inds=grain_phase[ids], phase_materials=(soft, hard), precision="mixed",
+                    solver_options={"line_search_flag": True})
 
 
 CONFIGS = {1: config1, 2: config2, 3: config3, 4: config4, 5: config5}
@@ -145,7 +150,8 @@ def run_steps(prob, w, params, sol, i0, i1, solver_options=None, on_step=None):
     Returns (params, sol, sigma_bar list (3x3 numpy per step))."""
     import torch
     from .solver import solver
-    opts = dict(solver_options or {})
+    opts = dict(w.solver_options)
+    opts.update(solver_options or {})
     out = []
     info = prob._bc_info
     for i in range(i0, i1):

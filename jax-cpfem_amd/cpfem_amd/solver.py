@@ -137,6 +137,11 @@ def solver(problem, solver_options={}):
         rel_res_val = res_val / res_val_initial
         newton += 1
         logger.debug(f"l_2 res = {res_val}, relative l_2 res = {rel_res_val}")
+    if not math.isfinite(res_val):
+        flags = int(max(status) if status else 0)
+        logger.error("non-finite residual after %d Newton iterations; point flags: %s", newton,
+                     ", ".join(n for b, n in ((N.PT_CAPPED, "local Newton hit its iteration cap"),
+                                              (N.PT_NONFINITE, "non-finite local state")) if flags & b) or "none")
     assert math.isfinite(res_val), "res_val contains NaN, stop the program!"
     assert bool(torch.isfinite(dofs).all()), "dofs contains NaN, stop the program!"
     problem.last_solve = dict(newton=newton, point_status=int(max(status) if status else 0),

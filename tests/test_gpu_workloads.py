@@ -1,0 +1,48 @@
+"""End-to-end parity of the benchmark workloads (small instances) against the oracle.
+
+Each case builds the workload with cpfem_amd.configs (same seeds as the fixture generator,
+tests/golden/make_oracle_fixtures.py), runs the reference driver loop through the drop-in
+solver on the device and compares the mean Cauchy stress of every load step with the
+oracle's.  Bar: 1e-8 relative per step in FP64 (Newton and Krylov stop at their own
+tolerances on slightly different rounding, so last-digit agreement is not expected); the
+mixed form against the same FP64 fixture at 1e-6.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+CASES = [("wl_c1_cu_8", 1, {}, 10), ("wl_c2_fcc_4", 2, {"n": 4}, 6), ("wl_c3_bcc24_4", 3, {"n": 4}, 4),
+         ("wl_c4_calib_4", 4, {"n": 4}, 4), ("wl_c5_dual_4", 5, {"n": 4}, 3)]
+
+
+def _run(cfg, kw, steps, precision):
+    from cpfem_amd import configs
+    w = configs.CONFIGS[cfg](**kw)
+    prob = configs.build_problem(w)
+    sol = [torch.zeros(prob.fes[0].num_total_nodes, 3, dtype=torch.float64, device="cuda")]
+    _, _, sig = configs.run_steps(prob, w, prob.internal_vars, sol, 0, steps, {"precision": precision})
+    return np.array([s.reshape(9) for s in sig])
+
+
+@pytest.mark.parametrize("name,cfg,kw,steps", CASES, ids=[c[0] for c in CASES])
+def test_workload_matches_oracle(name, cfg, kw, steps):
+    path = os.path.join(GOLDEN, name + ".txt")
+    if not os.path.exists(path):
+        pytest.skip(f"fixture {name}.txt not generated")
+    ref = np.loadtxt(path).reshape(steps, 9)
+    got = _run(cfg, kw, steps, "fp64")
+    scale = np.abs(ref).max(axis=1, keepdims=True)
+    assert (np.abs(got - ref) / scale).max() < 1e-8, (got[:, 8], ref[:, 8])
+
+
+def test_mixed_precision_workload():
+    ref = np.loadtxt(os.path.join(GOLDEN, "wl_c3_bcc24_4.txt")).reshape(4, 9)
+    got = _run(3, {"n": 4}, 4, "mixed")
+    scale = np.abs(ref).max(axis=1, keepdims=True)
+    assert (np.abs(got - ref) / scale).max() < 1e-6, (got[:, 8], ref[:, 8])
