@@ -116,16 +116,21 @@ def cpu_baseline(prob, w, params, sol, per_step, sample_cells=512, krylov_iters=
                         f"est. {step_s:.1f} s per load step"))
 
 
-def adjoint_bench(n=None, steps=None, k=10):
+def adjoint_bench(n=16, steps=None, k=10, fd=True, mixed=True):
     """Gradient of a calibration loss w.r.t. five hardening scales through the whole load
     history (hand-written adjoint, history checkpointed every k steps).  Workload: the
-    24^3 / 64-grain calibration case, loss = sum over steps of (szz - target)^2 with the
-    target from the hidden parameters and the query = hidden x (1 + U(-0.2, 0.2)) (seeded)."""
+    calibration case (64 grains, 304-like stiffness, hidden (g0, h0, gs, a, m) =
+    (90, 450, 280, 2.2, 0.02), 5% in 50 steps, the steel calibration driver's solver options),
+    loss = sum over steps of (szz - target)^2 with the target from the hidden parameters and
+    the query = hidden x (1 + U(-0.2, 0.2)) (seeded).  Default mesh 16^3: at 24^3 the forward
+    solve of the hidden parameters itself fails in load step 2 (a point's local Newton
+    overflows), so no gradient exists there.  Also reports a central difference (relative step
+    1e-6) of the g0 component and the gradient of the mixed-precision forward and adjoint."""
     import numpy as np
     import torch
     import cpfem_amd as C
     from cpfem_amd import configs
-    w = configs.config4(**({} if n is None else {"n": n}))
+    w = configs.config4(n=n)
     if steps is not None:
         w.disps, w.ts = w.disps[:steps + 1], w.ts[:steps + 1]
     prob = configs.build_problem(w)
@@ -139,26 +144,53 @@ def adjoint_bench(n=None, steps=None, k=10):
         sc = torch.stack([theta[4], theta[3], theta[1], theta[2], one, one, one, one])
         return (m.g0 * theta[0]).expand(nc, 8, Ns), (base * sc).expand(nc, 8, 8)
 
+    opts = dict(w.solver_options)
+
+    def loss_of(theta, target, precision="fp64"):
+        g0, m8 = inputs(theta)
+        o = dict(opts, precision=precision)
+        szz = C.simulate(prob, load, g0, m8, R, checkpoint_every=k, solver_options=o,
+                         adjoint_solver_options={"precision": precision})[:, 2, 2]
+        return ((szz - target) ** 2).sum() if target is not None else szz
+
     hidden = torch.ones(5, dtype=torch.float64, device="cuda")
     rng = np.random.default_rng(0)
-    query = torch.as_tensor(1.0 + rng.uniform(-0.2, 0.2, size=5), device="cuda").requires_grad_(True)
-    opts = dict(w.solver_options)
+    q0 = torch.as_tensor(1.0 + rng.uniform(-0.2, 0.2, size=5), device="cuda")
     with torch.no_grad():
-        g0, m8 = inputs(hidden)
-        target = C.simulate(prob, load, g0, m8, R, checkpoint_every=k, solver_options=opts)[:, 2, 2]
+        target = loss_of(hidden, None)
+    query = q0.clone().requires_grad_(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    g0, m8 = inputs(query)
-    szz = C.simulate(prob, load, g0, m8, R, checkpoint_every=k, solver_options=opts)[:, 2, 2]
-    loss = ((szz - target) ** 2).sum()
+    loss = loss_of(query, target)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     loss.backward()
     torch.cuda.synchronize()
     t2 = time.perf_counter()
-    return dict(workload=w.name, load_steps=load.n_steps, checkpoint_every=k, n_qp=w.n_qp,
-                forward_s=t1 - t0, backward_s=t2 - t1, grad_s=t2 - t0, loss=float(loss),
-                grad=[float(v) for v in query.grad.cpu()])
+    out = dict(workload=w.name, load_steps=load.n_steps, checkpoint_every=k, n_qp=w.n_qp,
+               forward_s=t1 - t0, backward_s=t2 - t1, grad_s=t2 - t0, loss=float(loss),
+               grad=[float(v) for v in query.grad.cpu()])
+    if fd:
+        h = 1e-6 * float(q0[0])
+        with torch.no_grad():
+            qp, qm = q0.clone(), q0.clone()
+            qp[0] += h
+            qm[0] -= h
+            fdv = (float(loss_of(qp, target)) - float(loss_of(qm, target))) / (2 * h)
+        out["fd_g0"] = fdv
+        out["fd_g0_rel_err"] = abs(out["grad"][0] - fdv) / abs(fdv)
+    if mixed:
+        qx = q0.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        lx = loss_of(qx, target, "mixed")
+        lx.backward()
+        torch.cuda.synchronize()
+        gx = qx.grad.cpu().numpy()
+        g64 = np.array(out["grad"])
+        out["mixed"] = dict(grad_s=time.perf_counter() - t3, loss_rel_err=abs(float(lx) - out["loss"]) / abs(out["loss"]),
+                            grad_max_rel_err=float(np.max(np.abs(gx - g64)) / np.max(np.abs(g64))))
+    return out
 
 
 def main():
@@ -295,6 +327,7 @@ def main():
     if args.adjoint:
         log("[bench] adjoint: calibration-workload history gradient ...")
         detail["adjoint"] = adjoint_bench(steps=args.adjoint_steps)
+        log(f"[bench] adjoint: {json.dumps(detail['adjoint'])}")
         log(f"[bench] adjoint: forward {detail['adjoint']['forward_s']:.2f}s, backward {detail['adjoint']['backward_s']:.2f}s")
 
     cpu = None

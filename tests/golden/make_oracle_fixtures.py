@@ -28,7 +28,7 @@ from cpfem_amd.rotations import get_rot_mat  # noqa: E402
 
 # name -> (builder, load steps)
 CASES = {
-    "wl_c1_cu_8": (lambda: configs.config1(), 10),
+    "wl_c1_cu_8": (lambda: configs.config1(steps=40), 10),  # 10 of 40 steps (10 steps of 1e-3 overflow)
     "wl_c2_fcc_4": (lambda: configs.config2(n=4), 6),
     "wl_c3_bcc24_4": (lambda: configs.config3(n=4), 4),
     "wl_c4_calib_4": (lambda: configs.config4(n=4), 4),

@@ -17,7 +17,7 @@ from conftest import GOLDEN
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
-CASES = [("wl_c1_cu_8", 1, {}, 10), ("wl_c2_fcc_4", 2, {"n": 4}, 6), ("wl_c3_bcc24_4", 3, {"n": 4}, 4),
+CASES = [("wl_c1_cu_8", 1, {"steps": 40}, 10), ("wl_c2_fcc_4", 2, {"n": 4}, 6), ("wl_c3_bcc24_4", 3, {"n": 4}, 4),
          ("wl_c4_calib_4", 4, {"n": 4}, 4), ("wl_c5_dual_4", 5, {"n": 4}, 3)]
 
 
