@@ -32,10 +32,31 @@ namespace cpfem {
 constexpr int KBS = 256;
 constexpr int KMAXB = 512;  // blocks of the BiCGSTAB kernels (grid-stride beyond)
 
+// One DPP step on a double: the value of the lane selected by CTRL (0 where the source is
+// out of range or the row is masked off).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_from(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, true);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over the 64 lanes of a wave, returned in every lane.  DPP row shifts 1, 2, 4, 8 leave
+// each 16-lane row's sum in its lane 15, row broadcasts 15 and 31 carry those into lane 63;
+// the order of the additions is fixed, so every wave that sums the same values gets the
+// bitwise-same result.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  return v;
+  v += dpp_from<0x111, 0xf>(v);  // row_shr:1
+  v += dpp_from<0x112, 0xf>(v);  // row_shr:2
+  v += dpp_from<0x114, 0xf>(v);  // row_shr:4
+  v += dpp_from<0x118, 0xf>(v);  // row_shr:8
+  v += dpp_from<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_from<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, 63);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
 // Block sum of NS values; result valid in thread 0.
