@@ -113,7 +113,7 @@ struct cpfem_ctx {
   // Krylov
   KrylovScalars* d_ksc = nullptr;
   double* d_kpartial = nullptr;  // [PS_COUNT][krylov blocks]
-  double* kv64[12] = {};  // r rhat pp q phat s shat t invd (spare) rh qh
+  double* kv64[12] = {};  // r rhat (spare) q phat s shat t invd (spare) rp rq; rp, rq hold 2 n
   float* d_mat32 = nullptr;
   double *d_r64 = nullptr, *d_tmp64 = nullptr;
   // pinned host staging
@@ -373,7 +373,7 @@ static int ensure_krylov(cpfem_ctx* c, bool mixed) {
   if (!c->d_ksc) {
     if ((rc = dalloc(&c->d_ksc, 1)) || (rc = dalloc(&c->d_kpartial, (size_t)8 * krylov_stride(n)))) return rc;
     for (int i = 0; i < 12; ++i)
-      if ((rc = dalloc(&c->kv64[i], n))) return rc;
+      if ((rc = dalloc(&c->kv64[i], i >= 10 ? 2 * n : n))) return rc;
     if ((rc = dalloc(&c->d_r64, n)) || (rc = dalloc(&c->d_tmp64, n))) return rc;
     HIPCHK(hipMemset(c->d_ksc, 0, sizeof(KrylovScalars)));
   }
@@ -396,15 +396,15 @@ static KVec kvec(double** v, double* x) {
   k.x = x;
   k.r = v[0];
   k.rhat = v[1];
-  k.pp = v[2];
+
   k.q = v[3];
   k.phat = v[4];
   k.s = v[5];
   k.shat = v[6];
   k.t = v[7];
   k.invd = v[8];
-  k.rh = v[10];
-  k.qh = v[11];
+  k.rp = reinterpret_cast<double2*>(v[10]);
+  k.rq = reinterpret_cast<double2*>(v[11]);
   return k;
 }
 

@@ -152,9 +152,12 @@ struct KrylovWork {
 
 // BiCGSTAB vectors.  Besides JAX's x, r, rhat, q, phat = M p, s, shat = M s, t the
 // iteration keeps rh = M r, qh = M q and pp = M (p - omega q), so that the next phat =
-// rh + beta pp and shat = rh - alpha qh can be formed inside the SpMV kernels.
+// rh + beta pp and shat = rh - alpha qh can be formed inside the SpMV kernels.  They are
+// stored as interleaved pairs rp = (rh, pp) and rq = (rh, qh), so that each SpMV gathers its
+// two operand vectors with one 16-byte load per column entry.
 struct KVec {
-  double *x, *r, *rhat, *pp, *q, *phat, *s, *shat, *t, *rh, *qh;
+  double *x, *r, *rhat, *q, *phat, *s, *shat, *t;
+  double2 *rp, *rq;
   const double* invd;
 };
 

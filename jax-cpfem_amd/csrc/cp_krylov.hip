@@ -245,8 +245,9 @@ __global__ void __launch_bounds__(KBS) k_kinit(BellView M, const TM* __restrict_
     const double r = b[i] - spmv_row<TM>(M, vals, v.x, i);
     v.r[i] = r;
     v.rhat[i] = r;
-    v.rh[i] = r * v.invd[i];
-    v.pp[i] = 0.0;  // M (p0 - omega0 q0) with p0 = q0 = r0, omega0 = 1
+    const double rh = r * v.invd[i];
+    v.rp[i] = make_double2(rh, 0.0);  // pp = M (p0 - omega0 q0) = 0 with p0 = q0 = r0, omega0 = 1
+    v.rq[i].x = rh;
     rr += r * r;
     bb += b[i] * b[i];
   }
@@ -315,9 +316,8 @@ struct SlotCols {
 
 // a1 += A[r, part's slots] u, a2 += A[r, part's slots] v.
 template <typename TM>
-__device__ __forceinline__ void spmv2_27(const BellView& M, const TM* __restrict__ vals, const double* __restrict__ u,
-                                         const double* __restrict__ v, const SlotCols& jc, int64_t r, int part, double& a1,
-                                         double& a2) {
+__device__ __forceinline__ void spmv2_27(const BellView& M, const TM* __restrict__ vals, const double2* __restrict__ uv,
+                                         const SlotCols& jc, int64_t r, int part, double& a1, double& a2) {
   const int64_t nrow = 3 * M.n_nodes;
 #pragma unroll
   for (int k = 0; k < SP_NS; ++k) {
@@ -328,16 +328,17 @@ __device__ __forceinline__ void spmv2_27(const BellView& M, const TM* __restrict
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const double m = (double)a[c * nrow];
-        a1 += m * u[jj + c];
-        a2 += m * v[jj + c];
+        const double2 x = uv[jj + c];
+        a1 += m * x.x;
+        a2 += m * x.y;
       }
     }
   }
 }
 
 template <typename TM>
-__device__ __forceinline__ void spmv2_any(const BellView& M, const TM* __restrict__ vals, const double* __restrict__ u,
-                                          const double* __restrict__ v, int64_t r, int part, double& a1, double& a2) {
+__device__ __forceinline__ void spmv2_any(const BellView& M, const TM* __restrict__ vals, const double2* __restrict__ uv,
+                                          int64_t r, int part, double& a1, double& a2) {
   const int64_t nrow = 3 * M.n_nodes;
   const int64_t i = r / 3;
   const int ns = M.nslot[i];
@@ -347,8 +348,9 @@ __device__ __forceinline__ void spmv2_any(const BellView& M, const TM* __restric
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const double m = (double)a[c * nrow];
-      a1 += m * u[jj + c];
-      a2 += m * v[jj + c];
+      const double2 x = uv[jj + c];
+      a1 += m * x.x;
+      a2 += m * x.y;
     }
   }
 }
@@ -375,17 +377,16 @@ __device__ __forceinline__ void part_sums_read(double& a1, double& a2, const Spm
 
 // One row group: (A u, A v) for rows r0 + 64 group + lane, complete in the part-0 waves.
 template <typename TM>
-__device__ __forceinline__ void spmv2_group(const BellView& M, const TM* __restrict__ vals, const double* __restrict__ u,
-                                            const double* __restrict__ v, int64_t r, int64_t n, const SpmvLane& t, double* red,
-                                            double& a1, double& a2) {
+__device__ __forceinline__ void spmv2_group(const BellView& M, const TM* __restrict__ vals, const double2* __restrict__ uv,
+                                            int64_t r, int64_t n, const SpmvLane& t, double* red, double& a1, double& a2) {
   a1 = a2 = 0.0;
   if (r < n) {
     if (M.n_slots == 27) {
       SlotCols jc;
       jc.load(M, r, t.part);
-      spmv2_27<TM>(M, vals, u, v, jc, r, t.part, a1, a2);
+      spmv2_27<TM>(M, vals, uv, jc, r, t.part, a1, a2);
     } else {
-      spmv2_any<TM>(M, vals, u, v, r, t.part, a1, a2);
+      spmv2_any<TM>(M, vals, uv, r, t.part, a1, a2);
     }
   }
   part_sums_write(a1, a2, t, red);
@@ -421,9 +422,8 @@ __device__ __forceinline__ void finish_first(double (&p)[NSL], double& a1, doubl
 // First row group of an SpMV kernel: column indices issued before the done test.  Returns
 // false if the solve is finished (uniform over the grid).
 template <typename TM>
-__device__ __forceinline__ bool first_group(const BellView& M, const TM* __restrict__ vals, const double* __restrict__ u,
-                                            const double* __restrict__ v, int64_t r, int64_t n, const SpmvLane& t, int done,
-                                            double& a1, double& a2) {
+__device__ __forceinline__ bool first_group(const BellView& M, const TM* __restrict__ vals, const double2* __restrict__ uv,
+                                            int64_t r, int64_t n, const SpmvLane& t, int done, double& a1, double& a2) {
   a1 = a2 = 0.0;
   const bool s27 = M.n_slots == 27;
   SlotCols jc;
@@ -431,9 +431,9 @@ __device__ __forceinline__ bool first_group(const BellView& M, const TM* __restr
   if (done) return false;
   if (r < n) {
     if (s27)
-      spmv2_27<TM>(M, vals, u, v, jc, r, t.part, a1, a2);
+      spmv2_27<TM>(M, vals, uv, jc, r, t.part, a1, a2);
     else
-      spmv2_any<TM>(M, vals, u, v, r, t.part, a1, a2);
+      spmv2_any<TM>(M, vals, uv, r, t.part, a1, a2);
   }
   return true;
 }
@@ -450,20 +450,19 @@ __global__ void __launch_bounds__(SBS) k_kpq(BellView M, const TM* __restrict__ 
   const int done = sc->done;
   const double rho = sc->rho2[pv], alpha = sc->alpha2[pv], omega = sc->omega2[pv], atol2 = sc->atol2;
   const int kprev = sc->k2[pv], maxiter = sc->maxiter;
-  const double* __restrict__ rh = v.rh;
-  const double* __restrict__ pp = v.pp;
+  const double2* __restrict__ rp = v.rp;
   const SpmvLane t;
   const int64_t n = w.n, step = (int64_t)gridDim.x * SP_BLOCK_ROWS;
   int64_t r = (int64_t)blockIdx.x * SP_BLOCK_ROWS + t.group * SP_ROWS + t.lane;
-  double e0 = 0.0, e1 = 0.0, e2 = 0.0, e3 = 0.0;  // rhat, M, rh, pp of the first row
+  double e0 = 0.0, e1 = 0.0;             // rhat, M of the first row
+  double2 e2 = make_double2(0.0, 0.0);  // (rh, pp) of the first row
   if (t.part == 0 && r < n) {
     e0 = v.rhat[r];
     e1 = v.invd[r];
-    e2 = rh[r];
-    e3 = pp[r];
+    e2 = rp[r];
   }
   double a1, a2;
-  if (!first_group<TM>(M, vals, rh, pp, r, n, t, done, a1, a2)) return;
+  if (!first_group<TM>(M, vals, rp, r, n, t, done, a1, a2)) return;
   double p[2] = {0.0, 0.0};
   for (int i = threadIdx.x; i < w.ps; i += SBS) {
     p[0] += w.partial[PS_RR * w.ps + i];
@@ -490,19 +489,18 @@ __global__ void __launch_bounds__(SBS) k_kpq(BellView M, const TM* __restrict__ 
       if (!first) {
         e0 = v.rhat[r];
         e1 = v.invd[r];
-        e2 = rh[r];
-        e3 = pp[r];
+        e2 = rp[r];
       }
       const double q = __fma_rn(beta, a2, a1);
-      v.phat[r] = __fma_rn(beta, e3, e2);
+      v.phat[r] = __fma_rn(beta, e2.y, e2.x);
       v.q[r] = q;
-      v.qh[r] = q * e1;
+      v.rq[r].y = q * e1;
       d += e0 * q;
     }
     first = false;
     r += step;
     if (r - t.group * SP_ROWS - t.lane >= n) break;  // uniform over the block
-    spmv2_group<TM>(M, vals, rh, pp, r, n, t, red, a1, a2);
+    spmv2_group<TM>(M, vals, rp, r, n, t, red, a1, a2);
   }
   put_partials<SBS, 1>({d}, {PS_RQ}, w.partial, w.ps, sh);
 }
@@ -516,20 +514,19 @@ __global__ void __launch_bounds__(SBS) k_kst(BellView M, const TM* __restrict__ 
   KrylovScalars* sc = w.sc;
   const int done = sc->done;
   const double rho = sc->rho2[par];
-  const double* __restrict__ rh = v.rh;
-  const double* __restrict__ qh = v.qh;
+  const double2* __restrict__ rq = v.rq;
   const SpmvLane t;
   const int64_t n = w.n, step = (int64_t)gridDim.x * SP_BLOCK_ROWS;
   int64_t r = (int64_t)blockIdx.x * SP_BLOCK_ROWS + t.group * SP_ROWS + t.lane;
-  double e0 = 0.0, e1 = 0.0, e2 = 0.0, e3 = 0.0;  // r, q, rh, qh of the first row
+  double e0 = 0.0, e1 = 0.0;             // r, q of the first row
+  double2 e2 = make_double2(0.0, 0.0);  // (rh, qh) of the first row
   if (t.part == 0 && r < n) {
     e0 = v.r[r];
     e1 = v.q[r];
-    e2 = rh[r];
-    e3 = qh[r];
+    e2 = rq[r];
   }
   double a1, a2;
-  if (!first_group<TM>(M, vals, rh, qh, r, n, t, done, a1, a2)) return;
+  if (!first_group<TM>(M, vals, rq, r, n, t, done, a1, a2)) return;
   double p[1] = {0.0};
   for (int i = threadIdx.x; i < w.ps; i += SBS) p[0] += w.partial[PS_RQ * w.ps + i];
   finish_first<1>(p, a1, a2, t, sh, red);
@@ -542,13 +539,12 @@ __global__ void __launch_bounds__(SBS) k_kst(BellView M, const TM* __restrict__ 
       if (!first) {
         e0 = v.r[r];
         e1 = v.q[r];
-        e2 = rh[r];
-        e3 = qh[r];
+        e2 = rq[r];
       }
       const double tv = __fma_rn(-alpha, a2, a1);
       const double s = e0 - alpha * e1;
       v.s[r] = s;
-      v.shat[r] = __fma_rn(-alpha, e3, e2);
+      v.shat[r] = __fma_rn(-alpha, e2.y, e2.x);
       v.t[r] = tv;
       ts += tv * s;
       tt += tv * tv;
@@ -557,7 +553,7 @@ __global__ void __launch_bounds__(SBS) k_kst(BellView M, const TM* __restrict__ 
     first = false;
     r += step;
     if (r - t.group * SP_ROWS - t.lane >= n) break;
-    spmv2_group<TM>(M, vals, rh, qh, r, n, t, red, a1, a2);
+    spmv2_group<TM>(M, vals, rq, r, n, t, red, a1, a2);
   }
   put_partials<SBS, 3>({ts, tt, ss}, {PS_TS, PS_TT, PS_SS}, w.partial, w.ps, sh);
 }
@@ -573,7 +569,7 @@ __global__ void __launch_bounds__(KBS) k_kxr(KVec v, KrylovWork w, int par) {
   double x = 0, ph = 0, shv = 0, s = 0, t = 0, qh = 0, rhat = 0, invd = 0;
   if (i0 < w.n) {  // issued before the done test (one round trip for both)
     x = v.x[i0]; ph = v.phat[i0]; shv = v.shat[i0]; s = v.s[i0]; t = v.t[i0];
-    qh = v.qh[i0]; rhat = v.rhat[i0]; invd = v.invd[i0];
+    qh = v.rq[i0].y; rhat = v.rhat[i0]; invd = v.invd[i0];
   }
   if (done) return;
   double sums[3];
@@ -586,7 +582,7 @@ __global__ void __launch_bounds__(KBS) k_kxr(KVec v, KrylovWork w, int par) {
   for (int64_t i = i0; i < w.n; i += (int64_t)w.nb * KBS) {
     if (i != i0) {
       x = v.x[i]; ph = v.phat[i]; shv = v.shat[i]; s = v.s[i]; t = v.t[i];
-      qh = v.qh[i]; rhat = v.rhat[i]; invd = v.invd[i];
+      qh = v.rq[i].y; rhat = v.rhat[i]; invd = v.invd[i];
     }
     double r;
     if (early) {
@@ -597,8 +593,9 @@ __global__ void __launch_bounds__(KBS) k_kxr(KVec v, KrylovWork w, int par) {
       r = s - omega * t;
     }
     v.r[i] = r;
-    v.rh[i] = r * invd;
-    v.pp[i] = __fma_rn(-omega, qh, ph);
+    const double rh = r * invd;
+    v.rp[i] = make_double2(rh, __fma_rn(-omega, qh, ph));
+    v.rq[i].x = rh;
     rr += r * r;
     rhr += rhat * r;
   }
@@ -615,7 +612,9 @@ __global__ void __launch_bounds__(KBS) k_kreplace(BellView M, const double* __re
   for (int64_t i = (int64_t)blockIdx.x * KBS + threadIdx.x; i < w.n; i += (int64_t)w.nb * KBS) {
     const double r = b[i] - spmv_row<double>(M, vals, v.x, i);
     v.r[i] = r;
-    v.rh[i] = r * v.invd[i];
+    const double rh = r * v.invd[i];
+    v.rp[i].x = rh;
+    v.rq[i].x = rh;
     rr += r * r;
     rhr += v.rhat[i] * r;
   }
