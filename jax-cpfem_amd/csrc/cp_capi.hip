@@ -35,6 +35,18 @@ int prof_begin(Prof* p, int cls, hipStream_t st) {
   return (int)p->pend.size() - 1;
 }
 
+bool prof_sample(Prof* p, int cls, hipEvent_t* start, hipEvent_t* stop) {
+  if (!p || !p->on) return false;
+  const int64_t k = p->launches[cls]++;
+  if (k % p->sample) return false;
+  if (p->pend.size() >= 8192) prof_flush(p);
+  Prof::Pending e{cls, prof_event(p), prof_event(p)};
+  p->pend.push_back(e);
+  *start = e.a;
+  *stop = e.b;
+  return true;
+}
+
 void prof_end(Prof* p, int slot, hipStream_t st) {
   if (!p || slot < 0) return;
   (void)hipEventRecord(p->pend[slot].b, st);

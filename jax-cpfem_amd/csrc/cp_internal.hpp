@@ -123,6 +123,10 @@ struct Prof {
 // Returns the event pair slot to record around the launch, or -1 (not sampled).
 int prof_begin(Prof* p, int cls, hipStream_t st);
 void prof_end(Prof* p, int slot, hipStream_t st);
+// For launches through hipExtLaunchKernelGGL, whose start/stop events time the kernel itself
+// (without the dispatch gap that stream-recorded events include): true and the event pair
+// if this launch of class cls is sampled.
+bool prof_sample(Prof* p, int cls, hipEvent_t* start, hipEvent_t* stop);
 int prof_flush(Prof* p);  // synchronises pending events into ms[]
 
 // ---------------------------------------------------------------- Krylov

@@ -23,6 +23,8 @@
 // kernel adds in a fixed order (bitwise reproducible run to run, no atomics).  All scalars
 // stay on the device; each kernel returns at once when the device-side `done` flag is set,
 // so the host enqueues iterations in batches and polls rarely.
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 
 #include "cp_internal.hpp"
@@ -661,19 +663,22 @@ void krylov_replace(const BellView& M, const double* vals64, const double* b, KV
   hipLaunchKernelGGL(k_kreplace, dim3((unsigned)w.nb), dim3(KBS), 0, st, M, vals64, b, v, w);
 }
 
+// Launch, timed with kernel-bound events when the profiler samples it.
+template <typename F, typename... Args>
+static inline void timed_launch(Prof* prof, int cls, F kernel, dim3 grid, dim3 block, hipStream_t st, Args... args) {
+  hipEvent_t a, b;
+  if (prof_sample(prof, cls, &a, &b))
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, st, a, b, 0, args...);
+  else
+    hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
+}
+
 template <typename TM>
 void krylov_iteration(const BellView& M, const TM* vals, KVec v, KrylovWork w, int par, hipStream_t st, Prof* prof) {
   const unsigned nb = (unsigned)w.nb, nbs = (unsigned)w.nbs;
-  int sl;
-  sl = prof_begin(prof, PC_SPMV, st);
-  hipLaunchKernelGGL(k_kpq<TM>, dim3(nbs), dim3(SBS), 0, st, M, vals, v, w, par);
-  prof_end(prof, sl, st);
-  sl = prof_begin(prof, PC_SPMV, st);
-  hipLaunchKernelGGL(k_kst<TM>, dim3(nbs), dim3(SBS), 0, st, M, vals, v, w, par);
-  prof_end(prof, sl, st);
-  sl = prof_begin(prof, PC_KVEC, st);
-  hipLaunchKernelGGL(k_kxr, dim3(nb), dim3(KBS), 0, st, v, w, par);
-  prof_end(prof, sl, st);
+  timed_launch(prof, PC_SPMV, k_kpq<TM>, dim3(nbs), dim3(SBS), st, M, vals, v, w, par);
+  timed_launch(prof, PC_SPMV, k_kst<TM>, dim3(nbs), dim3(SBS), st, M, vals, v, w, par);
+  timed_launch(prof, PC_KVEC, k_kxr, dim3(nb), dim3(KBS), st, v, w, par);
 }
 
 void launch_spmv(const BellView& M, const double* vals, const double* x, double* y, hipStream_t st) {
