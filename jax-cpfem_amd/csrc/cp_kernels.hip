@@ -232,7 +232,7 @@ __global__ void __launch_bounds__(256) k_element_R(Geom G, const double* __restr
 // ---------------------------------------------------------------------------------
 // Point-level update (tensor_map / update_int_vars_map, models_copper.py:263-269) on explicit
 // u_grad arrays [9][Q] (or from dofs when Hd == nullptr).  Any output pointer may be null.
-template <int N>
+template <int N, bool TANGENT>
 __global__ void __launch_bounds__(QBS) k_local(int64_t n_qp, const double* __restrict__ Hd, Geom G,
                                               const double* __restrict__ u, StateView sv, MatView mv, SlipTable<N> tab,
                                               SolveCfg cfg, int self_block, double* __restrict__ P_out,
@@ -260,7 +260,7 @@ __global__ void __launch_bounds__(QBS) k_local(int64_t n_qp, const double* __res
   const int it = pt.solve(tab, cfg.tol, cfg.max_sub_step, cfg.max_iter, s, &st);
   double P[9];
   pk1<N>(pt, F, Fpi, s, P);
-  if (A_out) {
+  if (TANGENT && A_out) {  // a separate instantiation: the tangent's registers cost the others occupancy
     tangent<N>(pt, tab, F, Fpi, s, [&](int m9, const double* dP) {
 #pragma unroll
       for (int i = 0; i < 9; ++i) A_out[(9 * i + m9) * n_qp + q] = dP[i];
@@ -333,8 +333,12 @@ void launch_local(int64_t n_qp, const double* Hd, const Geom& G, const double* u
                   const SlipTable<N>& tab, const SolveCfg& cfg, int self_block, double* P, double* A, double* sig,
                   const StateOut& so, int32_t* iters, int32_t* status, hipStream_t st) {
   const int64_t nb = (n_qp + QBS - 1) / QBS;
-  hipLaunchKernelGGL(k_local<N>, dim3((unsigned)nb), dim3(QBS), 0, st, n_qp, Hd, G, u, sv, mv, tab, cfg, self_block, P, A,
-                     sig, so, iters, status);
+  if (A)
+    hipLaunchKernelGGL((k_local<N, true>), dim3((unsigned)nb), dim3(QBS), 0, st, n_qp, Hd, G, u, sv, mv, tab, cfg, self_block,
+                       P, A, sig, so, iters, status);
+  else
+    hipLaunchKernelGGL((k_local<N, false>), dim3((unsigned)nb), dim3(QBS), 0, st, n_qp, Hd, G, u, sv, mv, tab, cfg,
+                       self_block, P, A, sig, so, iters, status);
 }
 
 void launch_element_K(const Geom& G, const double* wP, const double* wA, double* KE, double* RE, hipStream_t st) {
