@@ -303,11 +303,15 @@ struct SpmvLane {
   __device__ SpmvLane() : lane(threadIdx.x & 63), part((threadIdx.x >> 6) / SP_GROUPS), group((threadIdx.x >> 6) % SP_GROUPS) {}
 };
 
-// Column nodes of one part's share of a row on a 27-slot mesh.
+// Column nodes of one part's share of a row on a 27-slot mesh, and the row's slot count
+// (boundary nodes have fewer than 27 neighbours; their padding slots hold zeros and are not
+// read).
 struct SlotCols {
   int32_t j[SP_NS];
+  int ns;
   __device__ __forceinline__ void load(const BellView& M, int64_t r, int part) {
     const int64_t i = r / 3;
+    ns = M.nslot[i];
 #pragma unroll
     for (int k = 0; k < SP_NS; ++k) {
       const int s = part + SP_PARTS * k;
@@ -324,7 +328,7 @@ __device__ __forceinline__ void spmv2_27(const BellView& M, const TM* __restrict
 #pragma unroll
   for (int k = 0; k < SP_NS; ++k) {
     const int s = part + SP_PARTS * k;
-    if (s < 27) {
+    if (s < 27 && s < jc.ns) {
       const TM* a = vals + (3 * (int64_t)s) * nrow + r;
       const int64_t jj = 3 * (int64_t)jc.j[k];
 #pragma unroll
