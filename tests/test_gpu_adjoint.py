@@ -154,3 +154,64 @@ def test_dropin_calibration_gradient_matches_fd():
             am[k] -= h
             fd[k] = (simulation(ap) - simulation(am)).item() / (2 * h)
     assert np.abs(g - fd).max() / np.abs(fd).max() < 1e-5, (g, fd)
+
+
+def test_orientation_design_gradient_bcc24_dual_phase():
+    """Orientation-design scenario in small (poly_sensitivity_rot_mesh2.py:41-65,203-239 with the
+    dual-phase BCC24 model of models_DPsteel_inhomo.py): loss = (final mean szz - target)^2,
+    gradient w.r.t. per-grain z-y-x Euler angles through the whole history (checkpoint every 2
+    steps), FP64 and mixed precision, against central differences of the FP64 forward."""
+    import cpfem_amd as C
+    from cpfem_amd.rotations import alpha_rotate_tensor_rank_2, get_rot_mat, rotate_theta_transfer
+    rng = np.random.default_rng(7)
+    m = C.box_mesh(3, 3, 3, 1.0, 1.0, 1.0)
+    mesh = C.Mesh(m.points, m.cells_dict["hexahedron"])
+    nc, G = len(mesh.cells), 4
+    inds = rng.integers(0, G, size=nc)
+    phase = (inds == 3).astype(np.int64)  # one hard grain of four
+    soft = C.Material(slip="bcc24", g0=100.0, h=400.0, t_sat=250.0, gss_a=2.0, xm=0.05, r=1.4,
+                      C11=231.0e3, C12=135.0e3, C44=116.0e3, tol=1e-11)
+    left = lambda p: np.isclose(p[0], 0.0, atol=1e-5)
+    front = lambda p: np.isclose(p[1], 0.0, atol=1e-5)
+    bottom = lambda p: np.isclose(p[2], 0.0, atol=1e-5)
+    top = lambda p: np.isclose(p[2], 1.0, atol=1e-5)
+    zero = lambda p: 0.0
+    bc = lambda d: [[left, front, bottom, top], [0, 1, 2, 2], [zero, zero, zero, (lambda v: (lambda p: v))(d)]]
+    quat = C.rotations.uniform_quaternions(G, rng)
+    prob = C.CrystalPlasticity(mesh, dirichlet_bc_info=bc(0.0), additional_info=(quat, inds), material=soft)
+    load = C.LoadHistory(np.linspace(0, 3e-3, 4), np.linspace(0, 3.0, 4), bc)
+    ph = torch.as_tensor(phase, device="cuda")
+    g0 = torch.where(ph[:, None, None] == 1, 4 * soft.g0, soft.g0).to(torch.float64).expand(nc, 8, 24).contiguous()
+    base = torch.tensor([soft.xm, soft.gss_a, soft.h, soft.t_sat, soft.r, soft.C11, soft.C12, soft.C44],
+                        dtype=torch.float64, device="cuda")
+    mat8 = base.expand(nc, 8, 8).clone()
+    mat8[ph == 1, :, 2] *= 2.0  # h0 x2 in the hard phase
+    Rq = torch.as_tensor(get_rot_mat(quat), device="cuda")
+    idx = torch.as_tensor(inds, device="cuda")
+
+    def final_szz(euler, precision="fp64"):
+        R = alpha_rotate_tensor_rank_2(rotate_theta_transfer(euler), Rq)[idx].unsqueeze(1).expand(nc, 8, 3, 3)
+        o = dict(TIGHT, precision=precision)
+        return C.simulate(prob, load, g0, mat8, R, checkpoint_every=2, solver_options=o,
+                          adjoint_solver_options=dict(ADJ, precision=precision))[-1, 2, 2]
+
+    euler0 = torch.as_tensor(rng.uniform(-30, 30, size=(G, 3)), device="cuda")
+    with torch.no_grad():
+        target = final_szz(euler0) * 1.01
+    grads = {}
+    for prec in ("fp64", "mixed"):
+        e = euler0.clone().requires_grad_(True)
+        ((final_szz(e, prec) - target) ** 2).backward()
+        grads[prec] = e.grad.cpu().numpy()
+    fd = np.zeros((G, 3))
+    with torch.no_grad():
+        for g in range(G):
+            for j in range(3):
+                h = 1e-4
+                ep, em = euler0.clone(), euler0.clone()
+                ep[g, j] += h
+                em[g, j] -= h
+                fd[g, j] = (((final_szz(ep) - target) ** 2) - ((final_szz(em) - target) ** 2)).item() / (2 * h)
+    scale = np.abs(fd).max()
+    assert np.abs(grads["fp64"] - fd).max() / scale < 1e-5, (grads["fp64"], fd)
+    assert np.abs(grads["mixed"] - grads["fp64"]).max() / scale < 1e-6, (grads["mixed"], grads["fp64"])
