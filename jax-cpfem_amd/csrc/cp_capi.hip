@@ -422,8 +422,8 @@ static KVec kvec(double** v, double* x) {
 
 // One BiCGSTAB run (JAX semantics) with the Jacobi vector already in kv64[8]; returns the
 // iteration counter (negative: breakdown code).
-// With vals64 != nullptr (mixed form) the recurrence residual is replaced by the true FP64
-// residual after every batch (reliable updates).
+// With vals64 != nullptr (mixed form) a replacement of the recurrence residual by the true
+// FP64 residual is offered after every batch (reliable updates, k_kreplace).
 template <typename TM>
 static int run_bicgstab(cpfem_ctx* c, hipStream_t st, const TM* vals, const double* b, double* x, double tol, double atol,
                         int maxiter, int32_t* k_out, const double* vals64 = nullptr) {
@@ -454,7 +454,7 @@ static int run_bicgstab(cpfem_ctx* c, hipStream_t st, const TM* vals, const doub
     for (int i = 0; i < CH; ++i) krylov_iteration<TM>(M, vals, v, w, i & 1, st, &c->prof);
     if (vals64) {
       const int sr = prof_begin(&c->prof, PC_SPMV, st);
-      krylov_replace(M, vals64, b, v, w, st);
+      krylov_replace(M, vals64, b, v, w, batch, st);
       prof_end(&c->prof, sr, st);
     }
     issued += CH;
@@ -744,9 +744,10 @@ int cpfem_solve(cpfem_ctx* c, void* stream, const double* mat, const double* b, 
     inf.outer = 1;
   } else {
     // One BiCGSTAB whose SpMVs read an FP32 copy of the matrix (vectors and scalars FP64),
-    // with the recurrence residual replaced by the FP64 residual b - A x every batch; a
-    // restart from the current x follows only if the true residual is still above
-    // max(tol |b|, atol) when the iteration stops.
+    // with the recurrence residual replaced by the FP64 residual b - A x whenever it has
+    // fallen 100x since the last replacement (offered after every batch); a restart from the
+    // current x follows only if the true residual is still above max(tol |b|, atol) when the
+    // iteration stops.
     const BellView M = c->bell();
     launch_cast_f32(c->mat_len(), mat, c->d_mat32, st);
     const double thr2 = std::max(tol * tol * bb, atol * atol);

@@ -135,6 +135,8 @@ int prof_flush(Prof* p);  // synchronises pending events into ms[]
 struct KrylovScalars {
   double rho2[2], alpha2[2], omega2[2];
   double rs;     // <r, r> at the last convergence test
+  double rs_ref2[2];  // <r, r> at the last replacement of the recurrence residual (mixed form),
+                      // parity slots over the replacement offers (offer j writes [j & 1])
   double bs;     // <b, b>
   double atol2;
   double tol, atol;
@@ -172,7 +174,7 @@ void krylov_iteration(const BellView& M, const TM* vals, KVec v, KrylovWork w, i
 int krylov_blocks(int64_t n);
 int krylov_spmv_blocks(int64_t n);
 inline int krylov_stride(int64_t n) { return krylov_blocks(n) > krylov_spmv_blocks(n) ? krylov_blocks(n) : krylov_spmv_blocks(n); }
-void krylov_replace(const BellView& M, const double* vals64, const double* b, KVec v, KrylovWork w, hipStream_t st);
+void krylov_replace(const BellView& M, const double* vals64, const double* b, KVec v, KrylovWork w, int offer, hipStream_t st);
 void launch_spmv(const BellView& M, const double* vals, const double* x, double* y, hipStream_t st);
 void launch_inv_diag(const BellView& M, const double* vals, const int32_t* dslot, double* invd, int precond, hipStream_t st);
 
