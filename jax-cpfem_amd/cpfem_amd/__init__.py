@@ -5,7 +5,7 @@
 All compute runs in libcpfem_hip.so (hand-written HIP kernels, C ABI in include/cpfem.h);
 this package is the host-side mirror of the reference's Python surface.
 """
-from .generate_mesh import Mesh, box_mesh, get_meshio_cell_type
+from .generate_mesh import Mesh, box_mesh, get_meshio_cell_type, read_gmsh
 from .materials import (CALIB_COPPER, CALIB_STEEL304, CALIB_TANTALUM, COPPER, PRESETS, SENS_ROT, STEEL304,
                         TANTALUM, Material)
 from .models import CrystalPlasticity
@@ -14,7 +14,7 @@ from .solver import solver
 from .adjoint import ad_wrapper, implicit_vjp, simulate, LoadHistory
 from .rotations import alpha_rotate_tensor_rank_2, get_rot_mat, rotate_theta_transfer
 
-__all__ = ["CrystalPlasticity", "Problem", "solver", "Mesh", "box_mesh", "get_meshio_cell_type", "Material",
+__all__ = ["CrystalPlasticity", "Problem", "solver", "Mesh", "box_mesh", "get_meshio_cell_type", "read_gmsh", "Material",
            "BlockEllMatrix", "COPPER", "TANTALUM", "STEEL304", "CALIB_COPPER", "CALIB_TANTALUM",
            "CALIB_STEEL304", "SENS_ROT", "PRESETS", "ad_wrapper", "implicit_vjp", "simulate", "LoadHistory",
            "get_rot_mat", "rotate_theta_transfer", "alpha_rotate_tensor_rank_2"]
