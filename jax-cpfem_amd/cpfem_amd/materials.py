@@ -74,8 +74,16 @@ CALIB_STEEL304 = STEEL304.replace(layout="9", self_block=False)
 # crystal_plasticity_OR_design/models_sens_rot.py:53-60,100-102; tol 1e-6 (:211), max_sub_step 10 (:230)
 SENS_ROT = COPPER.replace(layout="9", self_block=False, tol=1e-6, max_sub_step=10)
 
+# polycrystal_DPsteel/models_DPsteel_inhomo.py:76-102 (ferrite = phase 0, martensite = phase 1;
+# BCC24 with the self block of :262-266, tol 1e-8 :302, max_sub_step 5 :321); used as
+# custom_init(quat, cell_ori_inds, phase_inds, (DP_FERRITE, DP_MARTENSITE))
+DP_FERRITE = Material(slip="bcc24", g0=170.0, h=400.0, t_sat=2500.0, gss_a=4.0, xm=0.05, r=1.0,
+                      C11=2.314e5, C12=1.347e5, C44=1.164e5, layout="10")
+DP_MARTENSITE = DP_FERRITE.replace(g0=435.0, h=950.0, t_sat=5300.0, C11=4.174e5, C12=2.424e5, C44=2.111e5)
+
 PRESETS = dict(copper=COPPER, tantalum=TANTALUM, steel304=STEEL304, calib_copper=CALIB_COPPER,
-               calib_tantalum=CALIB_TANTALUM, calib_steel304=CALIB_STEEL304, sens_rot=SENS_ROT)
+               calib_tantalum=CALIB_TANTALUM, calib_steel304=CALIB_STEEL304, sens_rot=SENS_ROT,
+               dp_ferrite=DP_FERRITE, dp_martensite=DP_MARTENSITE)
 
 
 def cubic_stiffness(C11, C12, C44):

@@ -12,16 +12,23 @@ Sources:
   ta_compression_szz.txt  calibration/data/csv/calibration_case2/ss_curve_e-2.txt
   cu_objective.txt        calibration/data/csv/calibration_case1/obj_val.txt (alpha.txt:1 = 1.4 x6)
 
-The 304-steel polycrystal run of the reference (polycrystal_304steel.py), from its own files:
-  steel304_mesh16.msh.gz  polycrystal_304steel/data/neper/polycrystal_304steel/domain0_mesh16.msh
-                          (the Neper mesh the driver reads, gzip-compressed, content unchanged)
-  steel304_quat.txt       polycrystal_304steel/data/csv/polycrystal_304steel/quat.txt
-  steel304_vtu.npz        from the driver's output polycrystal_304steel/data/vtk/polycrystal_304steel/
-                          u_000..u_015.vtu (load steps 1-16): per-step cell means of sigma_xx,
-                          sigma_yy, sigma_zz, von Mises (float64 means of the float32 cell data),
-                          the full float32 cell fields of steps 1, 5, 10 and 16, the cell
-                          orientation indices and the connectivity (to cross-check mesh ingest).
-                          VTU arrays are decoded with xml.etree + base64 + zlib (no code runs).
+Three reference runs, from their own files (the Neper mesh each driver reads, gzip-compressed
+with content unchanged; its quaternion table; and the driver's VTU output):
+  steel304_*   polycrystal_304steel.py: data/neper/polycrystal_304steel/domain0_mesh16.msh,
+               data/csv/polycrystal_304steel/quat.txt, data/vtk/polycrystal_304steel/u_000..u_015.vtu
+  ta_*         singlecrystal_tantalum.py: data/neper/singlecrystal_tantalum/mesh10.msh,
+               data/vtk/singlecrystal_tantalum/u_000..u_049.vtu (identity orientation, no table).
+               The driver text names mesh16.msh, but the stored VTUs have 1000 cells and the
+               connectivity of mesh10.msh, so that is the mesh of the run they record.
+  dp_*         polycrystal_DPsteel_inhomo.py: data/neper/polycrystal_DPsteel/n1-id1-mesh10.msh,
+               data/csv/polycrystal_DPsteel/quat.txt,
+               data/vtk/polycrystal_DPsteel/u_inhomo_000..u_inhomo_049.vtu
+Each <case>_vtu.npz holds per-step cell means of sigma_xx, sigma_yy, sigma_zz and von Mises
+(float64 means of the float32 cell data), the full float32 cell fields of four steps, the
+nodal displacement "sol" of the same four steps, the per-cell orientation (and, for dp,
+phase indices and the cubic constants C11, C12, C44) the run used (the dual-phase driver draws
+both at random without a seed; the VTU records the draw), and the connectivity (to cross-check
+mesh ingest).  VTU arrays are decoded with xml.etree + base64 + zlib (no code runs).
 """
 import base64
 import gzip
@@ -46,8 +53,19 @@ FILES = {
 }
 
 
-STEEL = "polycrystal_304steel/data"
-STEEL_FULL_STEPS = (0, 4, 9, 15)
+CASES = {
+    "steel304": dict(mesh="polycrystal_304steel/data/neper/polycrystal_304steel/domain0_mesh16.msh",
+                     quat="polycrystal_304steel/data/csv/polycrystal_304steel/quat.txt",
+                     vtu="polycrystal_304steel/data/vtk/polycrystal_304steel/u_{:03d}.vtu", steps=16,
+                     full=(0, 4, 9, 15), cell_fields=("cell_ori_inds",)),
+    "ta": dict(mesh="singlecrystal_tantalum/data/neper/singlecrystal_tantalum/mesh10.msh", quat=None,
+               vtu="singlecrystal_tantalum/data/vtk/singlecrystal_tantalum/u_{:03d}.vtu", steps=50,
+               full=(0, 9, 24, 49), cell_fields=("cell_ori_inds",)),
+    "dp": dict(mesh="polycrystal_DPsteel/data/neper/polycrystal_DPsteel/n1-id1-mesh10.msh",
+               quat="polycrystal_DPsteel/data/csv/polycrystal_DPsteel/quat.txt",
+               vtu="polycrystal_DPsteel/data/vtk/polycrystal_DPsteel/u_inhomo_{:03d}.vtu", steps=50,
+               full=(0, 9, 24, 49), cell_fields=("cell_ori_inds", "phase_inds", "C11", "C12", "C44")),
+}
 
 
 def vtu_arrays(path):
@@ -68,26 +86,27 @@ def vtu_arrays(path):
     return out
 
 
-def steel304():
-    src = os.path.join(REF, STEEL, "neper/polycrystal_304steel/domain0_mesh16.msh")
-    with open(src, "rb") as f, open(os.path.join(HERE, "steel304_mesh16.msh.gz"), "wb") as raw, \
+def vtu_case(name, mesh, quat, vtu, steps, full, cell_fields):
+    with open(os.path.join(REF, mesh), "rb") as f, open(os.path.join(HERE, f"{name}_mesh.msh.gz"), "wb") as raw, \
             gzip.GzipFile(filename="", mode="wb", fileobj=raw, mtime=0) as g:
         g.write(f.read())
-    q = np.loadtxt(os.path.join(REF, STEEL, "csv/polycrystal_304steel/quat.txt"))
-    np.savetxt(os.path.join(HERE, "steel304_quat.txt"), q, fmt="%.17g")
-    means, full = [], []
-    for i in range(16):
-        a = vtu_arrays(os.path.join(REF, STEEL, f"vtk/polycrystal_304steel/u_{i:03d}.vtu"))
+    if quat is not None:
+        np.savetxt(os.path.join(HERE, f"{name}_quat.txt"), np.loadtxt(os.path.join(REF, quat)), fmt="%.17g")
+    means, fields, sols, extra = [], [], [], {}
+    for i in range(steps):
+        a = vtu_arrays(os.path.join(REF, vtu.format(i)))
         cols = [a["sigma_xx"], a["sigma_yy"], a["sigma_zz"], a["von_Mises_stress"]]
         means.append([c.astype(np.float64).mean() for c in cols])
-        if i in STEEL_FULL_STEPS:
-            full.append(np.stack(cols[:3], axis=1))
+        if i in full:
+            fields.append(np.stack(cols[:3], axis=1))
+            sols.append(a["sol"].reshape(-1, 3))
         if i == 0:
-            ori = a["cell_ori_inds"].astype(np.int64)
-            conn = a["connectivity"].reshape(-1, 8).astype(np.int32)
-    np.savez_compressed(os.path.join(HERE, "steel304_vtu.npz"), mean_sigma=np.array(means), full_steps=np.array(STEEL_FULL_STEPS),
-                        full_sigma=np.array(full, dtype=np.float32), cell_ori_inds=ori, connectivity=conn)
-    print("steel304: mean szz", np.array(means)[:, 2])
+            for k in cell_fields:
+                extra[k] = a[k].astype(np.int64 if k.endswith("_inds") else np.float64)
+            extra["connectivity"] = a["connectivity"].reshape(-1, 8).astype(np.int32)
+    np.savez_compressed(os.path.join(HERE, f"{name}_vtu.npz"), mean_sigma=np.array(means), full_steps=np.array(full),
+                        full_sigma=np.array(fields, dtype=np.float32), full_sol=np.array(sols), **extra)
+    print(f"{name}: mean szz", np.array(means)[:, 2])
 
 
 def main():
@@ -98,7 +117,8 @@ def main():
         fmt = "%d" if dst.startswith("slip") else "%.17g"
         np.savetxt(os.path.join(HERE, dst), a, fmt=fmt)
         print(dst, a.shape)
-    steel304()
+    for name, case in CASES.items():
+        vtu_case(name, **case)
 
 
 if __name__ == "__main__":

@@ -12,7 +12,7 @@ pytest.importorskip("torch")
 
 def test_reference_neper_mesh():
     from cpfem_amd import read_gmsh
-    m = read_gmsh(os.path.join(GOLDEN, "steel304_mesh16.msh.gz"))
+    m = read_gmsh(os.path.join(GOLDEN, "steel304_mesh.msh.gz"))
     cells = m.cells_dict["hexahedron"]
     assert m.points.shape == (17 ** 3, 3) and cells.shape == (16 ** 3, 8)
     np.testing.assert_allclose(m.points.max(axis=0), 0.016)
@@ -24,6 +24,18 @@ def test_reference_neper_mesh():
     ref = np.load(os.path.join(GOLDEN, "steel304_vtu.npz"))  # what the reference's meshio read gave
     np.testing.assert_array_equal(cells, ref["connectivity"])
     np.testing.assert_array_equal(grains - 1, ref["cell_ori_inds"])
+
+
+@pytest.mark.parametrize("case", ["ta", "dp"])
+def test_reference_runs_mesh_matches_their_output(case):
+    """The tantalum (mesh10.msh) and dual-phase (n1-id1-mesh10.msh) meshes read here have the
+    connectivity the reference's VTU output records."""
+    from cpfem_amd import read_gmsh
+    m = read_gmsh(os.path.join(GOLDEN, f"{case}_mesh.msh.gz"))
+    ref = np.load(os.path.join(GOLDEN, f"{case}_vtu.npz"))
+    np.testing.assert_array_equal(m.cells_dict["hexahedron"], ref["connectivity"])
+    assert m.points.shape == (11 ** 3, 3)
+    assert np.all(m.cell_data["gmsh:physical"][0] == 1)  # one Neper grain; orientations are per cell
 
 
 def test_small_file_renumbering_and_filtering(tmp_path):
