@@ -28,7 +28,8 @@ Each <case>_vtu.npz holds per-step cell means of sigma_xx, sigma_yy, sigma_zz an
 nodal displacement "sol" of the same four steps, the per-cell orientation (and, for dp,
 phase indices and the cubic constants C11, C12, C44) the run used (the dual-phase driver draws
 both at random without a seed; the VTU records the draw), and the connectivity (to cross-check
-mesh ingest).  VTU arrays are decoded with xml.etree + base64 + zlib (no code runs).
+mesh ingest); dp_vtu_skeleton.txt lists the element tags and attributes of
+u_inhomo_000.vtu (no payload), the layout cpfem_amd.utils.save_sol reproduces.  VTU arrays are decoded with xml.etree + base64 + zlib (no code runs).
 """
 import base64
 import gzip
@@ -119,6 +120,18 @@ def main():
         print(dst, a.shape)
     for name, case in CASES.items():
         vtu_case(name, **case)
+    vtu_skeleton(CASES["dp"]["vtu"].format(0), os.path.join(HERE, "dp_vtu_skeleton.txt"))
+
+
+def vtu_skeleton(src, dst):
+    """Element tags and attributes of one reference VTU, in document order (array payloads
+    dropped): the layout a compatible writer must reproduce."""
+    lines = []
+    for el in ET.parse(os.path.join(REF, src)).getroot().iter():
+        lines.append(el.tag + " " + " ".join(f'{k}="{v}"' for k, v in el.attrib.items()))
+    with open(dst, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print(dst, len(lines), "elements")
 
 
 if __name__ == "__main__":
