@@ -11,6 +11,8 @@ Sources:
   cu_tension_szz.txt      calibration/data/csv/calibration_case1/stress_strain_curve_copper.txt
   ta_compression_szz.txt  calibration/data/csv/calibration_case2/ss_curve_e-2.txt
   cu_objective.txt        calibration/data/csv/calibration_case1/obj_val.txt (alpha.txt:1 = 1.4 x6)
+  cu_calib_alpha.txt      calibration/data/csv/calibration_case1/alpha.txt (the L-BFGS-B points the
+                          calibration run evaluated; no objective is stored for the second)
 
 Three reference runs, from their own files (the Neper mesh each driver reads, gzip-compressed
 with content unchanged; its quaternion table; and the driver's VTU output):
@@ -51,6 +53,7 @@ FILES = {
     "cu_tension_szz.txt": "calibration/data/csv/calibration_case1/stress_strain_curve_copper.txt",
     "ta_compression_szz.txt": "calibration/data/csv/calibration_case2/ss_curve_e-2.txt",
     "cu_objective.txt": "calibration/data/csv/calibration_case1/obj_val.txt",
+    "cu_calib_alpha.txt": "calibration/data/csv/calibration_case1/alpha.txt",
 }
 
 

@@ -6,7 +6,8 @@
 (2) The drop-in AD path of the reference's calibration driver (calibration_case1_
     singleCrystalCopper_GB.py:176-246: ad_wrapper + compute_avg_stress + update_int_vars_gp
     under autograd) for the Cu single-element objective at alpha = 1.4: gradient w.r.t. the six
-    scale coefficients.
+    scale coefficients; and the L-BFGS-B loop of that driver (cpfem_amd.optimize) against the
+    points the reference's calibration run logged.
 Central differences use relative step 1e-6 (1e-4 degree for angles) with the Newton
 tolerances tightened so the forward is smooth to ~1e-12.  The Krylov tolerance is 1e-12
 relative (atol negligible): near 1e-13 Jacobi-BiCGSTAB on these tiny systems reaches the
@@ -102,7 +103,9 @@ def test_simulate_gradient_matches_fd():
     assert np.abs(ge - fde).max() / np.abs(fde).max() < 1e-5, (ge, fde)
 
 
-def test_dropin_calibration_gradient_matches_fd():
+def _cu_calibration():
+    """The objective of calibration_case1_singleCrystalCopper_GB.py:176-246 through the
+    drop-in AD path, as a function of the six scale coefficients."""
     import cpfem_amd as C
     ref = torch.as_tensor(np.loadtxt(os.path.join(GOLDEN, "cu_tension_szz.txt"))[:20], device="cuda")
     m = C.box_mesh(1, 1, 1, 1.0, 1.0, 1.0)
@@ -139,6 +142,11 @@ def test_dropin_calibration_gradient_matches_fd():
         s = torch.stack(out)
         return 1e6 * torch.sum((s[idx] - ref[idx]) ** 2) / torch.sum(ref[idx] ** 2)
 
+    return simulation
+
+
+def test_dropin_calibration_gradient_matches_fd():
+    simulation = _cu_calibration()
     alpha = torch.full((6,), 1.4, dtype=torch.float64, device="cuda", requires_grad=True)
     obj = simulation(alpha)
     expect = float(np.loadtxt(os.path.join(GOLDEN, "cu_objective.txt")))
@@ -215,3 +223,31 @@ def test_orientation_design_gradient_bcc24_dual_phase():
     scale = np.abs(fd).max()
     assert np.abs(grads["fp64"] - fd).max() / scale < 1e-5, (grads["fp64"], fd)
     assert np.abs(grads["mixed"] - grads["fp64"]).max() / scale < 1e-6, (grads["mixed"], grads["fp64"])
+
+
+def test_calibration_loop_replays_reference_first_step(tmp_path):
+    """cpfem_amd.optimize.minimize_lbfgsb around the same objective, from the driver's start
+    point 1.4 with bounds [0.4, 3.0] (calibration_case1_singleCrystalCopper_GB.py:388-409),
+    logs the points the reference's run logged (tests/golden/cu_calib_alpha.txt): the first,
+    and the second that L-BFGS-B picks from the first gradient (a box corner set by the
+    gradient's signs).  The reference stored no objective for the second point, so the run
+    here stops when that point is handed to the objective."""
+    from cpfem_amd.optimize import minimize_lbfgsb
+    simulation = _cu_calibration()
+    calls = []
+
+    class Enough(Exception):
+        pass
+
+    def sim(x):
+        calls.append(x.detach().cpu().numpy().copy())
+        if len(calls) == 2:
+            raise Enough
+        return simulation(x)
+
+    with pytest.raises(Enough):
+        minimize_lbfgsb(sim, [1.4] * 6, ([0.4] * 6, [3.0] * 6), maxiter=10, obj_stop=1.0, log_dir=str(tmp_path))
+    ref_alpha = np.loadtxt(os.path.join(GOLDEN, "cu_calib_alpha.txt"))
+    np.testing.assert_array_equal(np.loadtxt(tmp_path / "alpha.txt"), ref_alpha)
+    expect = float(np.loadtxt(os.path.join(GOLDEN, "cu_objective.txt")))
+    assert abs(float(np.loadtxt(tmp_path / "obj_val.txt")) - expect) / expect < 1e-8
