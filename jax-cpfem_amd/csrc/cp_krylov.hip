@@ -185,11 +185,13 @@ __global__ void __launch_bounds__(KBS) k_inv_diag(BellView M, const double* __re
 // ((j + 1) & 1), which the host passes as `par`.
 enum { PS_RR = 0, PS_RHR, PS_RQ, PS_SS, PS_TS, PS_TT, PS_BB, PS_COUNT };
 
-// Writes this block's NSL sums into entry blockIdx.x of their slots; a grid smaller than
-// the stride ps also clears the entries no block of it owns, so every consumer sums exactly
-// ps values.  sh: NSL doubles per wave of the block.
+// Writes this block's NSL sums into entry `own` (a permutation of the block index over the
+// grid; the block index by default) of their slots; a grid smaller than the stride ps also
+// clears the entries no block of it owns, so every consumer sums exactly ps values.  sh: NSL
+// doubles per wave of the block.
 template <int BS, int NSL>
-__device__ __forceinline__ void put_partials(const double (&v)[NSL], const int (&slot)[NSL], double* partial, int ps, double* sh) {
+__device__ __forceinline__ void put_partials(const double (&v)[NSL], const int (&slot)[NSL], double* partial, int ps, double* sh,
+                                             int own = -1) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
   for (int j = 0; j < NSL; ++j) {
@@ -202,7 +204,7 @@ __device__ __forceinline__ void put_partials(const double (&v)[NSL], const int (
     double t = 0.0;
 #pragma unroll
     for (int i = 0; i < BS / 64; ++i) t += sh[j * (BS / 64) + i];
-    partial[slot[j] * ps + blockIdx.x] = t;
+    partial[slot[j] * ps + (own < 0 ? (int)blockIdx.x : own)] = t;
   }
   for (int i = blockIdx.x + (threadIdx.x + 1) * gridDim.x; i < ps; i += BS * gridDim.x) {
 #pragma unroll
@@ -288,16 +290,37 @@ __global__ void __launch_bounds__(KBS) k_kinit2(KrylovWork w) {
 // reduction: a kernel loads its scalars and the first row's epilogue operands, issues its
 // column indices before it tests the device-side `done` flag, streams values and gathers,
 // and meanwhile loads the partial sums it reduces.  (Measured on the 32^3 matrix: this form
-// at 1024-thread blocks and 4 waves/SIMD beats both a single-accumulator form that reduces
-// first and smaller blocks at higher occupancy.)
+// at 1024-thread blocks, 4 waves/SIMD and one block per CU beats both a single-accumulator
+// form that reduces first and smaller blocks at higher occupancy.)
+//
+// Work split: the 256-row tiles are dealt out as contiguous runs, one run per block (a block
+// walks its run; the next tile's epilogue operands are loaded before its SpMV starts).  XCD
+// locality: the hardware hands workgroups to the eight XCDs round-robin (block b runs on XCD
+// b % 8, each XCD with its own L2), so block b takes run xcd_rank(b), which gives every XCD
+// one contiguous range of rows: its L2 then holds one slab of the gathered vector and its halo
+// instead of lines from the whole vector.  Partial sums are stored at the run index.
 constexpr int SP_ROWS = 64;
 constexpr int SP_PARTS = 4;
 constexpr int SP_GROUPS = 4;
 constexpr int SP_BLOCK_ROWS = SP_ROWS * SP_GROUPS;
 constexpr int SBS = SP_ROWS * SP_PARTS * SP_GROUPS;
-constexpr int SP_MAXB = KMAXB;
 constexpr int SP_NS = (27 + SP_PARTS - 1) / SP_PARTS;  // slots per part on 27-slot meshes
 constexpr int SP_RED = (SP_PARTS - 1) * SP_GROUPS * SP_ROWS;
+
+constexpr int N_XCD = 8;
+__device__ __forceinline__ int xcd_rank(int b, int nb) {
+  const int x = b % N_XCD, per = nb / N_XCD, rem = nb % N_XCD;
+  return x * per + min(x, rem) + b / N_XCD;
+}
+// Tiles [first, end) of run v of nb over n rows (every run is non-empty when nb <= tiles).
+struct TileRun {
+  int64_t first, end;
+  __device__ TileRun(int v, int nb, int64_t n) {
+    const int64_t tiles = (n + SP_BLOCK_ROWS - 1) / SP_BLOCK_ROWS;
+    first = (int64_t)v * tiles / nb;
+    end = (int64_t)(v + 1) * tiles / nb;
+  }
+};
 
 struct SpmvLane {
   int lane, part, group;
@@ -459,10 +482,13 @@ __global__ void __launch_bounds__(SBS) k_kpq(BellView M, const TM* __restrict__ 
   const int kprev = sc->k2[pv], maxiter = sc->maxiter;
   const double2* __restrict__ rp = v.rp;
   const SpmvLane t;
-  const int64_t n = w.n, step = (int64_t)gridDim.x * SP_BLOCK_ROWS;
-  int64_t r = (int64_t)blockIdx.x * SP_BLOCK_ROWS + t.group * SP_ROWS + t.lane;
-  double e0 = 0.0, e1 = 0.0;             // rhat, M of the first row
-  double2 e2 = make_double2(0.0, 0.0);  // (rh, pp) of the first row
+  const int64_t n = w.n;
+  const int run = xcd_rank(blockIdx.x, gridDim.x);
+  const TileRun tiles(run, gridDim.x, n);
+  int64_t tile = tiles.first;
+  int64_t r = tile * SP_BLOCK_ROWS + t.group * SP_ROWS + t.lane;
+  double e0 = 0.0, e1 = 0.0;             // rhat, M of the row
+  double2 e2 = make_double2(0.0, 0.0);  // (rh, pp) of the row
   if (t.part == 0 && r < n) {
     e0 = v.rhat[r];
     e1 = v.invd[r];
@@ -490,26 +516,24 @@ __global__ void __launch_bounds__(SBS) k_kpq(BellView M, const TM* __restrict__ 
   if (!go) return;
   const double beta = rho_ / rho * alpha / omega;
   double d = 0.0;
-  bool first = true;
   for (;;) {
     if (t.part == 0 && r < n) {
-      if (!first) {
-        e0 = v.rhat[r];
-        e1 = v.invd[r];
-        e2 = rp[r];
-      }
       const double q = __fma_rn(beta, a2, a1);
       v.phat[r] = __fma_rn(beta, e2.y, e2.x);
       v.q[r] = q;
       v.rq[r].y = q * e1;
       d += e0 * q;
     }
-    first = false;
-    r += step;
-    if (r - t.group * SP_ROWS - t.lane >= n) break;  // uniform over the block
+    if (++tile >= tiles.end) break;  // uniform over the block
+    r += SP_BLOCK_ROWS;
+    if (t.part == 0 && r < n) {
+      e0 = v.rhat[r];
+      e1 = v.invd[r];
+      e2 = rp[r];
+    }
     spmv2_group<TM>(M, vals, rp, r, n, t, red, a1, a2);
   }
-  put_partials<SBS, 1>({d}, {PS_RQ}, w.partial, w.ps, sh);
+  put_partials<SBS, 1>({d}, {PS_RQ}, w.partial, w.ps, sh, run);
 }
 
 // t = A shat with shat = rh - alpha qh, alpha = rho / <rhat,q>.  Stores s = r - alpha q, shat,
@@ -523,10 +547,13 @@ __global__ void __launch_bounds__(SBS) k_kst(BellView M, const TM* __restrict__ 
   const double rho = sc->rho2[par];
   const double2* __restrict__ rq = v.rq;
   const SpmvLane t;
-  const int64_t n = w.n, step = (int64_t)gridDim.x * SP_BLOCK_ROWS;
-  int64_t r = (int64_t)blockIdx.x * SP_BLOCK_ROWS + t.group * SP_ROWS + t.lane;
-  double e0 = 0.0, e1 = 0.0;             // r, q of the first row
-  double2 e2 = make_double2(0.0, 0.0);  // (rh, qh) of the first row
+  const int64_t n = w.n;
+  const int run = xcd_rank(blockIdx.x, gridDim.x);
+  const TileRun tiles(run, gridDim.x, n);
+  int64_t tile = tiles.first;
+  int64_t r = tile * SP_BLOCK_ROWS + t.group * SP_ROWS + t.lane;
+  double e0 = 0.0, e1 = 0.0;             // r, q of the row
+  double2 e2 = make_double2(0.0, 0.0);  // (rh, qh) of the row
   if (t.part == 0 && r < n) {
     e0 = v.r[r];
     e1 = v.q[r];
@@ -540,14 +567,8 @@ __global__ void __launch_bounds__(SBS) k_kst(BellView M, const TM* __restrict__ 
   const double alpha = rho / p[0];
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->alpha2[par] = alpha;
   double ts = 0.0, tt = 0.0, ss = 0.0;
-  bool first = true;
   for (;;) {
     if (t.part == 0 && r < n) {
-      if (!first) {
-        e0 = v.r[r];
-        e1 = v.q[r];
-        e2 = rq[r];
-      }
       const double tv = __fma_rn(-alpha, a2, a1);
       const double s = e0 - alpha * e1;
       v.s[r] = s;
@@ -557,12 +578,16 @@ __global__ void __launch_bounds__(SBS) k_kst(BellView M, const TM* __restrict__ 
       tt += tv * tv;
       ss += s * s;
     }
-    first = false;
-    r += step;
-    if (r - t.group * SP_ROWS - t.lane >= n) break;
+    if (++tile >= tiles.end) break;  // uniform over the block
+    r += SP_BLOCK_ROWS;
+    if (t.part == 0 && r < n) {
+      e0 = v.r[r];
+      e1 = v.q[r];
+      e2 = rq[r];
+    }
     spmv2_group<TM>(M, vals, rq, r, n, t, red, a1, a2);
   }
-  put_partials<SBS, 3>({ts, tt, ss}, {PS_TS, PS_TT, PS_SS}, w.partial, w.ps, sh);
+  put_partials<SBS, 3>({ts, tt, ss}, {PS_TS, PS_TT, PS_SS}, w.partial, w.ps, sh, run);
 }
 
 // omega, x and r updates; stores rh = M r, pp = phat - omega qh and the partials of <r,r>,
@@ -671,8 +696,21 @@ void krylov_init(const BellView& M, const TM* vals, const double* b, KVec v, Kry
 }
 
 int krylov_blocks(int64_t n) { return (int)std::min<int64_t>(nblk(n), KMAXB); }
+// One SpMV block per CU of the current device (a 1024-thread block at this kernel's register
+// use fills a CU, so a larger grid runs in two rounds; fewer tiles than CUs: one per tile).
+// Measured on the 32^3 matrix (422 tiles): 256 blocks on 256 CUs, 46.3 us per BiCGSTAB
+// iteration; 384 blocks 61.7 us; 422 blocks 50.4 us.
 int krylov_spmv_blocks(int64_t n) {
-  return (int)std::max<int64_t>(1, std::min<int64_t>((n + SP_ROWS * SP_GROUPS - 1) / (SP_ROWS * SP_GROUPS), SP_MAXB));
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+  if (cus[dev] == 0) {
+    int c = 0;
+    if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0) c = 256;
+    cus[dev] = c;
+  }
+  const int64_t tiles = (n + SP_BLOCK_ROWS - 1) / SP_BLOCK_ROWS;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(tiles, cus[dev]));
 }
 
 void krylov_replace(const BellView& M, const double* vals64, const double* b, KVec v, KrylovWork w, int offer,
