@@ -41,7 +41,8 @@ enum {
   CPFEM_ELOCAL = 3,     /* some point hit the local iteration cap */
   CPFEM_EKRYLOV = 4,    /* |A x - b| >= 0.1 after the Krylov solve (solver.py:45) */
   CPFEM_ENONFINITE = 5, /* NaN / Inf in a residual */
-  CPFEM_ENOMEM = 6
+  CPFEM_ENOMEM = 6,
+  CPFEM_EDIRECT = 7     /* zero / non-finite pivot or non-finite result in the direct solve */
 };
 
 enum { CPFEM_FP64 = 0, CPFEM_MIXED = 1 };
@@ -146,6 +147,21 @@ int cpfem_update_state(cpfem_ctx* ctx, void* stream, const double* dofs, const C
    |A x - b| >= 0.1 as the reference asserts. */
 int cpfem_solve(cpfem_ctx* ctx, void* stream, const double* mat, const double* b, double* x, double tol,
                 double atol, int32_t maxiter, int32_t precond, int32_t precision, CpfemSolveInfo* info);
+
+/* Direct solve of A x = b (replaces umfpack_solve, crystal_plasticity_OR_design/solver.py:50-60,
+   selected by solver_options 'umfpack_solver' at :100-101): LU of the band form of A under a
+   bandwidth-reducing node order (unpivoted; a zero pivot returns CPFEM_EDIRECT), then `refine`
+   (0..10) steps of FP64 iterative refinement.  x need not hold a guess.  The band storage,
+   n_dofs x (4 * (3 * node bandwidth + 2) + 1) doubles under 64 GiB, is allocated on first use
+   and kept with the context.  info->residual = |A x - b|, info->iterations = refine. */
+int cpfem_solve_direct(cpfem_ctx* ctx, void* stream, const double* mat, const double* b, double* x, int32_t refine,
+                       CpfemSolveInfo* info);
+
+/* Node order of the direct solve, computed on the host (no device needed): the mesh's own
+   order or reverse Cuthill-McKee, whichever has the smaller node bandwidth.  col/nslot: the
+   block-ELL pattern ([n_slots][n_nodes] node columns, slot counts, as cpfem_pattern returns);
+   ip[old node] = new node; *bandwidth = max |ip[i] - ip[j]| over coupled nodes. */
+int cpfem_band_order(int64_t n_nodes, const int32_t* col, const int32_t* nslot, int32_t* ip, int32_t* bandwidth);
 
 /* y = A x on the block-ELL matrix (device). */
 int cpfem_spmv(cpfem_ctx* ctx, void* stream, const double* mat, const double* x, double* y);

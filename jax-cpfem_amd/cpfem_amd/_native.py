@@ -9,7 +9,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcpfem_hip.so")
 
-OK, EBADARG, EHIP, ELOCAL, EKRYLOV, ENONFINITE, ENOMEM = range(7)
+OK, EBADARG, EHIP, ELOCAL, EKRYLOV, ENONFINITE, ENOMEM, EDIRECT = range(8)
 FP64, MIXED = 0, 1
 APPLY_BC = 1
 XM, GSS_A, H, TSAT, R, C11, C12, C44, NPARAM = range(9)
@@ -77,6 +77,8 @@ _SIGS = {
     "cpfem_solve": (ctypes.c_int, [vp, vp, vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_int32,
                                    ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(CpfemSolveInfo)]),
     "cpfem_spmv": (ctypes.c_int, [vp, vp, vp, vp, vp]),
+    "cpfem_solve_direct": (ctypes.c_int, [vp, vp, vp, vp, vp, ctypes.c_int32, ctypes.POINTER(CpfemSolveInfo)]),
+    "cpfem_band_order": (ctypes.c_int, [ctypes.c_int64, vp, vp, vp, vp]),
     "cpfem_norm": (ctypes.c_int, [vp, vp, vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32)]),
     "cpfem_axpby": (ctypes.c_int, [vp, vp, ctypes.c_double, vp, ctypes.c_double, vp]),
     "cpfem_to_csr": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),

@@ -25,7 +25,7 @@ import torch
 from . import _native as N
 from .models import soa_empty, soa_ptr
 from .problem import device, ptr, stream_handle, to_device
-from .solver import krylov_solve, solver as _solver
+from .solver import krylov_solve, linear_solver, solver as _solver
 
 
 def _soa(t, nc, comps):
@@ -88,7 +88,9 @@ def adjoint_solve(problem, dofs, v, params=None, solver_options=None):
     _, mat = problem.assemble(dofs, params=params)
     matT = transpose(problem, mat)
     x0 = torch.zeros_like(v)
-    return krylov_solve(problem, matT, v.contiguous(), x0, solver_options or {})
+    # the same dispatch as the forward solve (solver.py:844 passes adjoint_solver_options to
+    # linear_solver), on a copy: linear_solver adds 'jax_solver' when no solver is named
+    return linear_solver(problem, matT, v.contiguous(), x0, dict(solver_options or {}))
 
 
 def _param_cotangents(problem, params, vj, nc):

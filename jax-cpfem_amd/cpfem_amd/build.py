@@ -1,6 +1,6 @@
 """Compile the HIP library libcpfem_hip.so for gfx950 in-tree (hipcc, no JIT cache).
 
-The three translation units are compiled in parallel and linked into the package
+The translation units are compiled in parallel and linked into the package
 directory, where ``cpfem_amd._native`` loads it.
 """
 import os
@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(os.path.dirname(HERE), "csrc")
 INCLUDE = os.path.join(os.path.dirname(os.path.dirname(HERE)), "include")
 LIB = os.path.join(HERE, "libcpfem_hip.so")
-SOURCES = ("cp_capi.hip", "cp_kernels.hip", "cp_krylov.hip", "cp_adjoint.hip")
+SOURCES = ("cp_capi.hip", "cp_kernels.hip", "cp_krylov.hip", "cp_adjoint.hip", "cp_direct.hip")
 ARCH = os.environ.get("CPFEM_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-I", INCLUDE,
          # packed FP32 instructions stay off (Build)

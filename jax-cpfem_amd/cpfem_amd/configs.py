@@ -163,8 +163,10 @@ def run_steps(prob, w, params, sol, i0, i1, solver_options=None, on_step=None):
         info[-1][-1] = (lambda d: (lambda p: d))(w.disps[i + 1])
         prob.fes[0].update_Dirichlet_boundary_conditions(info)
         prob.set_params(params)
-        o = {"jax_solver": {}, "initial_guess": sol}
+        o = {"initial_guess": sol}
         o.update(opts)
+        if not (o.keys() & {"jax_solver", "umfpack_solver", "petsc_solver", "custom_solver"}):
+            o["jax_solver"] = {}
         sol = solver(prob, o)
         sig = prob.compute_avg_stress(sol[0], params)
         out.append(sig.mean(dim=0).cpu().numpy())

@@ -133,6 +133,10 @@ struct cpfem_ctx {
   double* h_dbl = nullptr;
   int32_t* h_int = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
+  // banded direct solver (allocated on first use)
+  BandLU band;
+  int32_t* d_ip = nullptr;
+  int32_t* d_bflag = nullptr;
   Prof prof;
 
   Geom geom() const {
@@ -545,6 +549,7 @@ int cpfem_create(const CpfemMesh* mesh, const CpfemSlip* slip, cpfem_ctx** out) 
 void cpfem_destroy(cpfem_ctx* c) {
   if (!c) return;
   (void)hipDeviceSynchronize();
+  dfree(c->band.band); dfree(c->band.y); dfree(c->d_ip); dfree(c->d_bflag);
   dfree(c->d_cells); dfree(c->d_sg); dfree(c->d_jxw);
   dfree(c->d_col); dfree(c->d_nslot); dfree(c->d_dslot); dfree(c->d_tslot);
   dfree(c->d_cptr); dfree(c->d_clist); dfree(c->d_nptr); dfree(c->d_nlist);
@@ -692,6 +697,76 @@ int cpfem_update_state(cpfem_ctx* c, void* stream, const double* dofs, const Cpf
   if (out->Fp_inv == state->Fp_inv || out->g == state->g || out->slip == state->slip)
     return fail(CPFEM_EBADARG, "state_out must not alias state");
   return cpfem_local(c, stream, c->n_qp, nullptr, dofs, state, params, nullptr, nullptr, nullptr, out, nullptr);
+}
+
+int cpfem_band_order(int64_t n_nodes, const int32_t* col, const int32_t* nslot, int32_t* ip, int32_t* bandwidth) {
+  if (n_nodes <= 0 || !col || !nslot || !ip || !bandwidth) return fail(CPFEM_EBADARG, "null argument");
+  return band_ordering(n_nodes, col, nslot, ip, bandwidth);
+}
+
+// Band storage is capped (BAND_MAX_BYTES); larger systems are refused with a message that
+// names the Krylov solver.
+static constexpr double BAND_MAX_BYTES = 64.0 * (1ull << 30);
+
+static int ensure_band(cpfem_ctx* c) {
+  if (c->band.band) return CPFEM_OK;
+  std::vector<int32_t> ip(c->n_nodes);
+  int32_t bw = 0;
+  band_ordering(c->n_nodes, c->h_col.data(), c->h_nslot.data(), ip.data(), &bw);
+  BandLU B;
+  B.n = c->n_dofs;
+  B.kl = B.ku = 3 * bw + 2;
+  B.W = B.kl + B.ku + 1;
+  const double bytes = (double)B.n * B.W * sizeof(double);
+  if (bytes > BAND_MAX_BYTES)
+    return fail(CPFEM_EBADARG, "direct solve: band storage of " + std::to_string((long long)(bytes / (1 << 20))) +
+                                   " MiB exceeds the limit; use the Krylov solver ('jax_solver')");
+  int rc;
+  if ((rc = dalloc(&c->d_ip, (size_t)c->n_nodes)) || (rc = dalloc(&c->d_bflag, 1)) || (rc = dalloc(&B.y, (size_t)B.n)) ||
+      (rc = dalloc(&B.band, (size_t)B.n * B.W)))
+    return rc;
+  HIPCHK(hipMemcpy(c->d_ip, ip.data(), ip.size() * 4, hipMemcpyHostToDevice));
+  B.ip = c->d_ip;
+  B.flag = c->d_bflag;
+  c->band = B;
+  return CPFEM_OK;
+}
+
+int cpfem_solve_direct(cpfem_ctx* c, void* stream, const double* mat, const double* b, double* x, int32_t refine,
+                       CpfemSolveInfo* info) {
+  if (!c || !mat || !b || !x) return fail(CPFEM_EBADARG, "null argument");
+  if (refine < 0 || refine > 10) return fail(CPFEM_EBADARG, "refine must be in 0..10");
+  int rc;
+  if ((rc = ensure_krylov(c, false)) || (rc = ensure_band(c))) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const BandLU& B = c->band;
+  CpfemSolveInfo inf{};
+  double bb;
+  if ((rc = dot_host(c, st, b, b, &bb))) return rc;
+  inf.b_norm = std::sqrt(bb);
+  band_assemble(B, c->bell(), mat, st);
+  band_factor(B, st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(c->h_int, c->d_bflag, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (c->h_int[0] != 0) return fail(CPFEM_EDIRECT, "direct solve: zero or non-finite pivot in the unpivoted band LU");
+  band_solve(B, b, x, st);
+  // x += A^-1 (b - A x), with the residual in FP64 on the block-ELL matrix
+  for (int k = 0; k < refine; ++k) {
+    launch_residual64(c->bell(), mat, x, b, c->d_r64, st);
+    band_solve(B, c->d_r64, c->d_tmp64, st);
+    launch_axpby(c->n_dofs, 1.0, c->d_tmp64, 1.0, x, st);
+  }
+  HIPCHK(hipGetLastError());
+  launch_residual64(c->bell(), mat, x, b, c->d_tmp64, st);
+  double err2;
+  if ((rc = dot_host(c, st, c->d_tmp64, c->d_tmp64, &err2))) return rc;
+  inf.residual = std::sqrt(err2);
+  inf.iterations = refine;
+  inf.outer = 1;
+  if (info) *info = inf;
+  if (!std::isfinite(inf.residual)) return fail(CPFEM_EDIRECT, "direct solve: non-finite solution");
+  return CPFEM_OK;
 }
 
 int cpfem_spmv(cpfem_ctx* c, void* stream, const double* mat, const double* x, double* y) {

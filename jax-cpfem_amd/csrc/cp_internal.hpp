@@ -167,6 +167,22 @@ struct KVec {
   const double* invd;
 };
 
+// Banded LU of the Newton matrix (direct-solve option): n dofs, half bandwidths kl = ku,
+// row-major band of width W = kl + ku + 1 under the node map ip (old -> new), work vector y,
+// pivot flag.
+struct BandLU {
+  int64_t n = 0;
+  int kl = 0, ku = 0, W = 0;
+  double* band = nullptr;
+  const int32_t* ip = nullptr;
+  double* y = nullptr;
+  int32_t* flag = nullptr;
+};
+int band_ordering(int64_t n_nodes, const int32_t* col, const int32_t* nslot, int32_t* ip, int32_t* bandwidth);
+void band_assemble(const BandLU& B, const BellView& M, const double* vals, hipStream_t st);
+void band_factor(const BandLU& B, hipStream_t st);
+void band_solve(const BandLU& B, const double* b, double* x, hipStream_t st);
+
 template <typename TM>
 void krylov_init(const BellView& M, const TM* vals, const double* b, KVec v, KrylovWork w, hipStream_t st);
 template <typename TM>
