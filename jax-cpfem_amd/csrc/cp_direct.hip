@@ -7,8 +7,8 @@
 // order or reverse Cuthill-McKee, whichever is narrower), factored in place as A = L U
 // without pivoting (right-looking, panels of BAND_NB columns: the diagonal block in one
 // wave's registers, then L21 = A21 U11^-1 and U12 = L11^-1 A12 one row / column per thread in
-// LDS, then an LDS-tiled trailing update A22 -= L21 U12), and solved with two one-block
-// triangular sweeps.  FP64 iterative refinement against the block-ELL matrix follows in the
+// LDS, then an LDS-tiled trailing update A22 -= L21 U12), and solved with two triangular
+// sweeps of two launches per panel.  FP64 iterative refinement against the block-ELL matrix follows in the
 // caller.  Without pivoting the factorisation exists when every leading principal minor is
 // non-zero, which holds for these stiffness matrices (elastic-plastic tangents close to
 // SPD, unit Dirichlet rows); a zero or non-finite pivot is reported instead of being worked
@@ -27,7 +27,6 @@ namespace cpfem {
 
 constexpr int BAND_NB = 32;     // panel width
 constexpr int BAND_TILE = 64;   // trailing-update tile (rows x columns)
-constexpr int BAND_PBS = 1024;  // threads of the one-block kernels
 
 // Element (i, j) of the band (valid when -kl <= j - i <= ku).
 __device__ __forceinline__ int64_t bidx(int64_t i, int64_t j, int kl, int W) { return i * W + (j - i + kl); }
@@ -89,19 +88,28 @@ __global__ void __launch_bounds__(256) k_band_lrow(double* __restrict__ band, in
     const int p = e / BAND_NB, q = e % BAND_NB;
     U[p][q] = (q >= p && q - p <= ku) ? band[bidx(k0 + p, k0 + q, kl, W)] : 0.0;
   }
-  const int t = threadIdx.x;
-  const int64_t i = k0 + BAND_NB + (int64_t)blockIdx.x * 256 + t;
-  const bool on = i < n && i <= k0 + BAND_NB - 1 + kl;
-  for (int q = 0; q < BAND_NB; ++q) A[q][t] = (on && i - (k0 + q) <= kl) ? band[bidx(i, k0 + q, kl, W)] : 0.0;
-  __syncthreads();
-  if (!on) return;
-  for (int p = 0; p < BAND_NB; ++p) {
-    double v = A[p][t];
-    for (int q = 0; q < p; ++q) v -= A[q][t] * U[q][p];
-    A[p][t] = v / U[p][p];
+  const int64_t i0 = k0 + BAND_NB + (int64_t)blockIdx.x * 256, ilast = std::min<int64_t>(n - 1, k0 + BAND_NB - 1 + kl);
+  // rows read and written as contiguous lines: consecutive threads take consecutive columns
+  for (int e = threadIdx.x; e < 256 * BAND_NB; e += 256) {
+    const int r = e / BAND_NB, q = e % BAND_NB;
+    const int64_t i = i0 + r;
+    A[q][r] = (i <= ilast && i - (k0 + q) <= kl) ? band[bidx(i, k0 + q, kl, W)] : 0.0;
   }
-  for (int q = 0; q < BAND_NB; ++q)
-    if (i - (k0 + q) <= kl) band[bidx(i, k0 + q, kl, W)] = A[q][t];
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (i0 + t <= ilast) {
+    for (int p = 0; p < BAND_NB; ++p) {
+      double v = A[p][t];
+      for (int q = 0; q < p; ++q) v -= A[q][t] * U[q][p];
+      A[p][t] = v / U[p][p];
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 256 * BAND_NB; e += 256) {
+    const int r = e / BAND_NB, q = e % BAND_NB;
+    const int64_t i = i0 + r;
+    if (i <= ilast && i - (k0 + q) <= kl) band[bidx(i, k0 + q, kl, W)] = A[q][r];
+  }
 }
 
 // U12 = L11^-1 A12: one thread per column right of the full panel, L11 (unit lower) and each
@@ -172,68 +180,61 @@ __global__ void __launch_bounds__(256) k_band_trail(double* __restrict__ band, i
   }
 }
 
-// y = L^-1 y (unit lower band), one block: per panel, the panel rows are solved by wave 0
-// (lane p holds row k0+p) and the rows below are updated by the whole block.
-__global__ void __launch_bounds__(BAND_PBS) k_band_fwd(const double* __restrict__ band, double* __restrict__ y, int64_t n,
-                                                      int kl, int W) {
-  __shared__ double yp[BAND_NB];
-  for (int64_t k0 = 0; k0 < n; k0 += BAND_NB) {
-    const int nb = (int)std::min<int64_t>(BAND_NB, n - k0);
-    if (threadIdx.x < 64) {
-      const int p = threadIdx.x;
-      double v = p < nb ? y[k0 + p] : 0.0;
-      for (int q = 0; q < nb; ++q) {
-        const double vq = __shfl(v, q);  // row k0+q is final here
-        if (p > q && p < nb && (k0 + p) - (k0 + q) <= kl) v -= band[bidx(k0 + p, k0 + q, kl, W)] * vq;
-      }
-      if (p < nb) {
-        y[k0 + p] = v;
-        yp[p] = v;
-      }
-    }
-    __syncthreads();
-    const int64_t iend = std::min<int64_t>(k0 + nb - 1 + kl, n - 1);
-    for (int64_t i = k0 + nb + threadIdx.x; i <= iend; i += BAND_PBS) {
-      double acc = 0.0;
-      for (int p = 0; p < nb; ++p)
-        if (i - (k0 + p) <= kl) acc += band[bidx(i, k0 + p, kl, W)] * yp[p];
-      y[i] -= acc;
-    }
-    __syncthreads();
+// Triangular sweeps, two launches per panel: one wave solves the panel rows (lane p holds
+// row k0+p with its panel row of L or U preloaded; rows are finalised in order and broadcast
+// by shuffles), then a grid over the rows the panel reaches subtracts the panel's
+// contribution (one thread per row, its BAND_NB loads issued together).
+__global__ void __launch_bounds__(64) k_band_fwd_diag(const double* __restrict__ band, double* __restrict__ y, int kl, int W,
+                                                     int64_t k0, int nb) {
+  const int p = threadIdx.x;
+  double lr[BAND_NB];
+#pragma unroll
+  for (int q = 0; q < BAND_NB; ++q) lr[q] = (q < p && p < nb && p - q <= kl) ? band[bidx(k0 + p, k0 + q, kl, W)] : 0.0;
+  double v = p < nb ? y[k0 + p] : 0.0;
+#pragma unroll
+  for (int q = 0; q < BAND_NB; ++q) {
+    const double vq = __shfl(v, q);  // row k0+q is final here
+    v -= lr[q] * vq;
   }
+  if (p < nb) y[k0 + p] = v;
 }
 
-// y = U^-1 y (upper band with diagonal), one block, panels from the last.
-__global__ void __launch_bounds__(BAND_PBS) k_band_bwd(const double* __restrict__ band, double* __restrict__ y, int64_t n,
-                                                      int kl, int ku, int W) {
-  __shared__ double yp[BAND_NB];
-  const int64_t npan = (n + BAND_NB - 1) / BAND_NB;
-  for (int64_t pi = npan - 1; pi >= 0; --pi) {
-    const int64_t k0 = pi * BAND_NB;
-    const int nb = (int)std::min<int64_t>(BAND_NB, n - k0);
-    if (threadIdx.x < 64) {
-      const int p = threadIdx.x;
-      double v = p < nb ? y[k0 + p] : 0.0;
-      for (int q = nb - 1; q >= 0; --q) {
-        if (p == q) v /= band[bidx(k0 + p, k0 + p, kl, W)];
-        const double vq = __shfl(v, q);  // row k0+q is final here
-        if (p < q && q - p <= ku) v -= band[bidx(k0 + p, k0 + q, kl, W)] * vq;
-      }
-      if (p < nb) {
-        y[k0 + p] = v;
-        yp[p] = v;
-      }
-    }
-    __syncthreads();
-    const int64_t ibeg = std::max<int64_t>(0, k0 - ku);
-    for (int64_t i = ibeg + threadIdx.x; i < k0; i += BAND_PBS) {
-      double acc = 0.0;
-      for (int p = 0; p < nb; ++p)
-        if ((k0 + p) - i <= ku) acc += band[bidx(i, k0 + p, kl, W)] * yp[p];
-      y[i] -= acc;
-    }
-    __syncthreads();
+__global__ void __launch_bounds__(256) k_band_fwd_upd(const double* __restrict__ band, double* __restrict__ y, int64_t n, int kl,
+                                                     int W, int64_t k0, int nb) {
+  const int64_t i = k0 + nb + (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n || i > k0 + nb - 1 + kl) return;
+  double acc = 0.0;
+#pragma unroll
+  for (int p = 0; p < BAND_NB; ++p)
+    if (p < nb && i - (k0 + p) <= kl) acc += band[bidx(i, k0 + p, kl, W)] * y[k0 + p];
+  y[i] -= acc;
+}
+
+__global__ void __launch_bounds__(64) k_band_bwd_diag(const double* __restrict__ band, double* __restrict__ y, int kl, int ku,
+                                                     int W, int64_t k0, int nb) {
+  const int p = threadIdx.x;
+  double ur[BAND_NB];
+#pragma unroll
+  for (int q = 0; q < BAND_NB; ++q) ur[q] = (q >= p && q < nb && q - p <= ku) ? band[bidx(k0 + p, k0 + q, kl, W)] : 0.0;
+  double v = p < nb ? y[k0 + p] : 0.0;
+#pragma unroll
+  for (int q = BAND_NB - 1; q >= 0; --q) {
+    if (p == q && p < nb) v /= ur[q];
+    const double vq = __shfl(v, q);  // row k0+q is final here
+    if (p < q) v -= ur[q] * vq;
   }
+  if (p < nb) y[k0 + p] = v;
+}
+
+__global__ void __launch_bounds__(256) k_band_bwd_upd(const double* __restrict__ band, double* __restrict__ y, int kl, int ku,
+                                                     int W, int64_t k0, int nb) {
+  const int64_t i = std::max<int64_t>(0, k0 - ku) + (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= k0) return;
+  double acc = 0.0;
+#pragma unroll
+  for (int p = 0; p < BAND_NB; ++p)
+    if (p < nb && (k0 + p) - i <= ku) acc += band[bidx(i, k0 + p, kl, W)] * y[k0 + p];
+  y[i] -= acc;
 }
 
 // out[new order] = in[old order] (dir = 0) or out[old] = in[new] (dir = 1), dof-wise.
@@ -364,8 +365,22 @@ void band_solve(const BandLU& B, const double* b, double* x, hipStream_t st) {
   const int64_t nn = B.n / 3;
   const unsigned g = (unsigned)((B.n + 255) / 256);
   hipLaunchKernelGGL(k_band_perm, dim3(g), dim3(256), 0, st, nn, B.ip, b, B.y, 0);
-  hipLaunchKernelGGL(k_band_fwd, dim3(1), dim3(BAND_PBS), 0, st, B.band, B.y, B.n, B.kl, B.W);
-  hipLaunchKernelGGL(k_band_bwd, dim3(1), dim3(BAND_PBS), 0, st, B.band, B.y, B.n, B.kl, B.ku, B.W);
+  for (int64_t k0 = 0; k0 < B.n; k0 += BAND_NB) {
+    const int nb = (int)std::min<int64_t>(BAND_NB, B.n - k0);
+    hipLaunchKernelGGL(k_band_fwd_diag, dim3(1), dim3(64), 0, st, B.band, B.y, B.kl, B.W, k0, nb);
+    const int64_t rows = std::min<int64_t>(B.kl, B.n - (k0 + nb));
+    if (rows > 0)
+      hipLaunchKernelGGL(k_band_fwd_upd, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, st, B.band, B.y, B.n, B.kl, B.W, k0,
+                         nb);
+  }
+  for (int64_t k0 = ((B.n - 1) / BAND_NB) * BAND_NB; k0 >= 0; k0 -= BAND_NB) {
+    const int nb = (int)std::min<int64_t>(BAND_NB, B.n - k0);
+    hipLaunchKernelGGL(k_band_bwd_diag, dim3(1), dim3(64), 0, st, B.band, B.y, B.kl, B.ku, B.W, k0, nb);
+    const int64_t rows = std::min<int64_t>(B.ku, k0);
+    if (rows > 0)
+      hipLaunchKernelGGL(k_band_bwd_upd, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, st, B.band, B.y, B.kl, B.ku, B.W, k0,
+                         nb);
+  }
   hipLaunchKernelGGL(k_band_perm, dim3(g), dim3(256), 0, st, nn, B.ip, B.y, x, 1);
 }
 
