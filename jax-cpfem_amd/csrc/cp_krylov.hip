@@ -495,12 +495,12 @@ __global__ void __launch_bounds__(SBS) k_kpq(BellView M, const TM* __restrict__ 
     e2 = rp[r];
   }
   double a1, a2;
-  if (!first_group<TM>(M, vals, rp, r, n, t, done, a1, a2)) return;
   double p[2] = {0.0, 0.0};
-  for (int i = threadIdx.x; i < w.ps; i += SBS) {
+  for (int i = threadIdx.x; i < w.ps; i += SBS) {  // issued before the SpMV: independent of it
     p[0] += w.partial[PS_RR * w.ps + i];
     p[1] += w.partial[PS_RHR * w.ps + i];
   }
+  if (!first_group<TM>(M, vals, rp, r, n, t, done, a1, a2)) return;
   finish_first<2>(p, a1, a2, t, sh, red);
   const double rs = p[0], rho_ = p[1];
   int k = (omega == 0.0 || alpha == 0.0) ? -11 : kprev + 1;
@@ -560,9 +560,9 @@ __global__ void __launch_bounds__(SBS) k_kst(BellView M, const TM* __restrict__ 
     e2 = rq[r];
   }
   double a1, a2;
-  if (!first_group<TM>(M, vals, rq, r, n, t, done, a1, a2)) return;
   double p[1] = {0.0};
-  for (int i = threadIdx.x; i < w.ps; i += SBS) p[0] += w.partial[PS_RQ * w.ps + i];
+  for (int i = threadIdx.x; i < w.ps; i += SBS) p[0] += w.partial[PS_RQ * w.ps + i];  // before the SpMV
+  if (!first_group<TM>(M, vals, rq, r, n, t, done, a1, a2)) return;
   finish_first<1>(p, a1, a2, t, sh, red);
   const double alpha = rho / p[0];
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->alpha2[par] = alpha;
