@@ -79,18 +79,22 @@ __global__ void __launch_bounds__(64) k_band_diag(double* __restrict__ band, int
     if (p < nb && q < nb && q - p <= ku && p - q <= kl) band[bidx(k0 + p, k0 + q, kl, W)] = a[q];
 }
 
+// Rows (columns) per block of the L21 (U12) kernels: small blocks spread a panel's ~kl rows
+// over many CUs.
+constexpr int LR_BS = 64;
+
 // L21 = A21 U11^-1: one thread per row below the full panel k0..k0+BAND_NB-1, U11 and each
 // thread's row of panel entries in LDS (register arrays of this size spill).
-__global__ void __launch_bounds__(256) k_band_lrow(double* __restrict__ band, int64_t n, int kl, int ku, int W, int64_t k0) {
+__global__ void __launch_bounds__(LR_BS) k_band_lrow(double* __restrict__ band, int64_t n, int kl, int ku, int W, int64_t k0) {
   __shared__ double U[BAND_NB][BAND_NB + 1];
-  __shared__ double A[BAND_NB][256 + 1];  // A[q][thread]
-  for (int e = threadIdx.x; e < BAND_NB * BAND_NB; e += 256) {
+  __shared__ double A[BAND_NB][LR_BS + 1];  // A[q][thread]
+  for (int e = threadIdx.x; e < BAND_NB * BAND_NB; e += LR_BS) {
     const int p = e / BAND_NB, q = e % BAND_NB;
     U[p][q] = (q >= p && q - p <= ku) ? band[bidx(k0 + p, k0 + q, kl, W)] : 0.0;
   }
-  const int64_t i0 = k0 + BAND_NB + (int64_t)blockIdx.x * 256, ilast = std::min<int64_t>(n - 1, k0 + BAND_NB - 1 + kl);
+  const int64_t i0 = k0 + BAND_NB + (int64_t)blockIdx.x * LR_BS, ilast = std::min<int64_t>(n - 1, k0 + BAND_NB - 1 + kl);
   // rows read and written as contiguous lines: consecutive threads take consecutive columns
-  for (int e = threadIdx.x; e < 256 * BAND_NB; e += 256) {
+  for (int e = threadIdx.x; e < LR_BS * BAND_NB; e += LR_BS) {
     const int r = e / BAND_NB, q = e % BAND_NB;
     const int64_t i = i0 + r;
     A[q][r] = (i <= ilast && i - (k0 + q) <= kl) ? band[bidx(i, k0 + q, kl, W)] : 0.0;
@@ -105,7 +109,7 @@ __global__ void __launch_bounds__(256) k_band_lrow(double* __restrict__ band, in
     }
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < 256 * BAND_NB; e += 256) {
+  for (int e = threadIdx.x; e < LR_BS * BAND_NB; e += LR_BS) {
     const int r = e / BAND_NB, q = e % BAND_NB;
     const int64_t i = i0 + r;
     if (i <= ilast && i - (k0 + q) <= kl) band[bidx(i, k0 + q, kl, W)] = A[q][r];
@@ -114,15 +118,15 @@ __global__ void __launch_bounds__(256) k_band_lrow(double* __restrict__ band, in
 
 // U12 = L11^-1 A12: one thread per column right of the full panel, L11 (unit lower) and each
 // thread's column of panel entries in LDS.
-__global__ void __launch_bounds__(256) k_band_urow(double* __restrict__ band, int64_t n, int kl, int ku, int W, int64_t k0) {
+__global__ void __launch_bounds__(LR_BS) k_band_urow(double* __restrict__ band, int64_t n, int kl, int ku, int W, int64_t k0) {
   __shared__ double L[BAND_NB][BAND_NB + 1];
-  __shared__ double U[BAND_NB][256 + 1];  // U[p][thread]
-  for (int e = threadIdx.x; e < BAND_NB * BAND_NB; e += 256) {
+  __shared__ double U[BAND_NB][LR_BS + 1];  // U[p][thread]
+  for (int e = threadIdx.x; e < BAND_NB * BAND_NB; e += LR_BS) {
     const int p = e / BAND_NB, q = e % BAND_NB;
     L[p][q] = (q < p && p - q <= kl) ? band[bidx(k0 + p, k0 + q, kl, W)] : 0.0;
   }
   const int t = threadIdx.x;
-  const int64_t j = k0 + BAND_NB + (int64_t)blockIdx.x * 256 + t;
+  const int64_t j = k0 + BAND_NB + (int64_t)blockIdx.x * LR_BS + t;
   const bool on = j < n && j <= k0 + BAND_NB - 1 + ku;
   for (int p = 0; p < BAND_NB; ++p) U[p][t] = (on && j - (k0 + p) <= ku) ? band[bidx(k0 + p, j, kl, W)] : 0.0;
   __syncthreads();
@@ -183,7 +187,7 @@ __global__ void __launch_bounds__(256) k_band_trail(double* __restrict__ band, i
 // Triangular sweeps, two launches per panel: one wave solves the panel rows (lane p holds
 // row k0+p with its panel row of L or U preloaded; rows are finalised in order and broadcast
 // by shuffles), then a grid over the rows the panel reaches subtracts the panel's
-// contribution (one thread per row, its BAND_NB loads issued together).
+// contribution.
 __global__ void __launch_bounds__(64) k_band_fwd_diag(const double* __restrict__ band, double* __restrict__ y, int kl, int W,
                                                      int64_t k0, int nb) {
   const int p = threadIdx.x;
@@ -199,15 +203,25 @@ __global__ void __launch_bounds__(64) k_band_fwd_diag(const double* __restrict__
   if (p < nb) y[k0 + p] = v;
 }
 
+// 8 lanes per row, 4 consecutive panel columns each (a row's 32 entries are one contiguous
+// 256-byte run), reduced with shuffles; 32 rows per 256-thread block, so a panel's rows spread
+// over many CUs.
+constexpr int UPD_LANES = 8;
 __global__ void __launch_bounds__(256) k_band_fwd_upd(const double* __restrict__ band, double* __restrict__ y, int64_t n, int kl,
                                                      int W, int64_t k0, int nb) {
-  const int64_t i = k0 + nb + (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n || i > k0 + nb - 1 + kl) return;
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t i = k0 + nb + g / UPD_LANES;
+  const int sub = (int)(g % UPD_LANES);
+  const bool on = i < n && i <= k0 + nb - 1 + kl;
   double acc = 0.0;
 #pragma unroll
-  for (int p = 0; p < BAND_NB; ++p)
-    if (p < nb && i - (k0 + p) <= kl) acc += band[bidx(i, k0 + p, kl, W)] * y[k0 + p];
-  y[i] -= acc;
+  for (int c = 0; c < BAND_NB / UPD_LANES; ++c) {
+    const int p = sub * (BAND_NB / UPD_LANES) + c;
+    if (on && p < nb && i - (k0 + p) <= kl) acc += band[bidx(i, k0 + p, kl, W)] * y[k0 + p];
+  }
+#pragma unroll
+  for (int off = UPD_LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, UPD_LANES);
+  if (on && sub == 0) y[i] -= acc;
 }
 
 __global__ void __launch_bounds__(64) k_band_bwd_diag(const double* __restrict__ band, double* __restrict__ y, int kl, int ku,
@@ -228,13 +242,19 @@ __global__ void __launch_bounds__(64) k_band_bwd_diag(const double* __restrict__
 
 __global__ void __launch_bounds__(256) k_band_bwd_upd(const double* __restrict__ band, double* __restrict__ y, int kl, int ku,
                                                      int W, int64_t k0, int nb) {
-  const int64_t i = std::max<int64_t>(0, k0 - ku) + (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= k0) return;
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t i = std::max<int64_t>(0, k0 - ku) + g / UPD_LANES;
+  const int sub = (int)(g % UPD_LANES);
+  const bool on = i < k0;
   double acc = 0.0;
 #pragma unroll
-  for (int p = 0; p < BAND_NB; ++p)
-    if (p < nb && (k0 + p) - i <= ku) acc += band[bidx(i, k0 + p, kl, W)] * y[k0 + p];
-  y[i] -= acc;
+  for (int c = 0; c < BAND_NB / UPD_LANES; ++c) {
+    const int p = sub * (BAND_NB / UPD_LANES) + c;
+    if (on && p < nb && (k0 + p) - i <= ku) acc += band[bidx(i, k0 + p, kl, W)] * y[k0 + p];
+  }
+#pragma unroll
+  for (int off = UPD_LANES / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, UPD_LANES);
+  if (on && sub == 0) y[i] -= acc;
 }
 
 // out[new order] = in[old order] (dir = 0) or out[old] = in[new] (dir = 1), dof-wise.
@@ -354,8 +374,10 @@ void band_factor(const BandLU& B, hipStream_t st) {
     if (rest <= 0) break;  // only the last panel is narrower than BAND_NB
     // rows k0+nb .. k0+nb-1+kl and columns k0+nb .. k0+nb-1+ku reach the panel
     const int64_t ncol = std::min<int64_t>(rest, B.ku), nrow = std::min<int64_t>(rest, B.kl);
-    hipLaunchKernelGGL(k_band_lrow, dim3((unsigned)((nrow + 255) / 256)), dim3(256), 0, st, B.band, B.n, B.kl, B.ku, B.W, k0);
-    hipLaunchKernelGGL(k_band_urow, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, st, B.band, B.n, B.kl, B.ku, B.W, k0);
+    hipLaunchKernelGGL(k_band_lrow, dim3((unsigned)((nrow + LR_BS - 1) / LR_BS)), dim3(LR_BS), 0, st, B.band, B.n, B.kl, B.ku,
+                       B.W, k0);
+    hipLaunchKernelGGL(k_band_urow, dim3((unsigned)((ncol + LR_BS - 1) / LR_BS)), dim3(LR_BS), 0, st, B.band, B.n, B.kl, B.ku,
+                       B.W, k0);
     const dim3 grid((unsigned)((ncol + BAND_TILE - 1) / BAND_TILE), (unsigned)((nrow + BAND_TILE - 1) / BAND_TILE));
     hipLaunchKernelGGL(k_band_trail, grid, dim3(256), 0, st, B.band, B.n, B.kl, B.ku, B.W, k0, nb);
   }
@@ -370,16 +392,16 @@ void band_solve(const BandLU& B, const double* b, double* x, hipStream_t st) {
     hipLaunchKernelGGL(k_band_fwd_diag, dim3(1), dim3(64), 0, st, B.band, B.y, B.kl, B.W, k0, nb);
     const int64_t rows = std::min<int64_t>(B.kl, B.n - (k0 + nb));
     if (rows > 0)
-      hipLaunchKernelGGL(k_band_fwd_upd, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, st, B.band, B.y, B.n, B.kl, B.W, k0,
-                         nb);
+      hipLaunchKernelGGL(k_band_fwd_upd, dim3((unsigned)((rows * UPD_LANES + 255) / 256)), dim3(256), 0, st, B.band, B.y, B.n,
+                         B.kl, B.W, k0, nb);
   }
   for (int64_t k0 = ((B.n - 1) / BAND_NB) * BAND_NB; k0 >= 0; k0 -= BAND_NB) {
     const int nb = (int)std::min<int64_t>(BAND_NB, B.n - k0);
     hipLaunchKernelGGL(k_band_bwd_diag, dim3(1), dim3(64), 0, st, B.band, B.y, B.kl, B.ku, B.W, k0, nb);
     const int64_t rows = std::min<int64_t>(B.ku, k0);
     if (rows > 0)
-      hipLaunchKernelGGL(k_band_bwd_upd, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, st, B.band, B.y, B.kl, B.ku, B.W, k0,
-                         nb);
+      hipLaunchKernelGGL(k_band_bwd_upd, dim3((unsigned)((rows * UPD_LANES + 255) / 256)), dim3(256), 0, st, B.band, B.y, B.kl,
+                         B.ku, B.W, k0, nb);
   }
   hipLaunchKernelGGL(k_band_perm, dim3(g), dim3(256), 0, st, nn, B.ip, B.y, x, 1);
 }
