@@ -193,6 +193,42 @@ def adjoint_bench(n=16, steps=None, k=10, fd=True, mixed=True):
     return out
 
 
+def direct_bench(n, reps):
+    """Banded direct solve (factor + solve + two refinement solves) on the row-eliminated
+    elastic Newton matrix of the n^3 FCC polycrystal, against the Krylov solve of the same
+    system; one JSON line."""
+    import torch
+    from cpfem_amd import configs
+    from cpfem_amd.solver import direct_solve, krylov_solve
+    torch.cuda.set_device(0)
+    w = configs.config2(n=n)
+    prob = configs.build_problem(w)
+    prob.fes[0].update_Dirichlet_boundary_conditions(w.dirichlet_bc_info(w.disps[1]))
+    prob.dt = w.ts[1] - w.ts[0]
+    prob.set_params(prob.internal_vars)
+    res, mat = prob.assemble(torch.zeros(prob.ctx.n_dofs, dtype=torch.float64, device="cuda"))
+    b = torch.randn(prob.ctx.n_dofs, dtype=torch.float64, device="cuda", generator=torch.Generator("cuda").manual_seed(0))
+    direct_solve(prob, mat, b)  # allocates the band storage
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(max(reps, 1)):
+        x = direct_solve(prob, mat, b)
+    torch.cuda.synchronize()
+    t_direct = (time.perf_counter() - t0) / max(reps, 1)
+    info = prob.solver_info[-1]
+    t0 = time.perf_counter()
+    xk = krylov_solve(prob, mat, b, torch.zeros_like(b))
+    torch.cuda.synchronize()
+    t_kry = time.perf_counter() - t0
+    print(json.dumps({
+        "metric": "direct solve time (banded LU + 2 refinement steps)", "value": round(t_direct * 1e3, 2), "unit": "ms",
+        "higher_is_better": False, "n_gpus": 1, "steps": max(reps, 1), "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"FCC polycrystal {n}^3 elastic Newton matrix", "n_dofs": prob.ctx.n_dofs},
+        "residual_rel": info["residual"] / info["b_norm"], "krylov_ms": round(t_kry * 1e3, 2),
+        "krylov_iterations": prob.solver_info[-1]["iterations"],
+        "diff_vs_krylov": float(torch.linalg.norm(x - xk) / torch.linalg.norm(x))}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -208,10 +244,16 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true", help="also run a plain copy kernel (counter scale)")
     ap.add_argument("--adjoint", action="store_true", help="also time the history gradient of the calibration workload")
     ap.add_argument("--adjoint-steps", type=int, default=None, help="load steps of the adjoint workload (default: all 50)")
+    ap.add_argument("--direct", action="store_true",
+                    help="instead of the load steps: time the banded direct solve ('umfpack_solver') on the "
+                         "elastic Newton matrix of the FCC polycrystal at --mesh (default 20)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.direct:
+        direct_bench(args.mesh or 20, args.steps)
+        return
 
     import numpy as np
     import torch
