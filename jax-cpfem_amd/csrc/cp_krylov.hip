@@ -425,6 +425,31 @@ __device__ __forceinline__ void spmv2_group(const BellView& M, const TM* __restr
   __syncthreads();
 }
 
+// spmv2_group with the row's 27-slot column indices already loaded (jc, valid for r < n).
+template <typename TM>
+__device__ __forceinline__ void spmv2_group_pre(const BellView& M, const TM* __restrict__ vals, const double2* __restrict__ uv,
+                                                const SlotCols& jc, int64_t r, int64_t n, const SpmvLane& t, double* red,
+                                                double& a1, double& a2) {
+  a1 = a2 = 0.0;
+  if (r < n) {
+    if (M.n_slots == 27)
+      spmv2_27<TM>(M, vals, uv, jc, r, t.part, a1, a2);
+    else
+      spmv2_any<TM>(M, vals, uv, r, t.part, a1, a2);
+  }
+  part_sums_write(a1, a2, t, red);
+  __syncthreads();
+  part_sums_read(a1, a2, t, red);
+  __syncthreads();
+}
+
+// Column indices of the row one tile further on, issued early (none past the block's run or
+// past the last row; ns = 0 then).
+__device__ __forceinline__ void load_next_cols(SlotCols& jn, const BellView& M, int64_t r_next, int64_t n, bool more, int part) {
+  jn.ns = 0;
+  if (more && M.n_slots == 27 && r_next < n) jn.load(M, r_next, part);
+}
+
 // Sums of NSL partial slots from registers already loaded (p[j]: this thread's share), with
 // the part sums of the first row group riding on the same barrier.
 template <int NSL>
@@ -501,6 +526,8 @@ __global__ void __launch_bounds__(SBS) k_kpq(BellView M, const TM* __restrict__ 
     p[1] += w.partial[PS_RHR * w.ps + i];
   }
   if (!first_group<TM>(M, vals, rp, r, n, t, done, a1, a2)) return;
+  SlotCols jn;  // the next tile's columns, in flight during the reductions and the epilogue
+  load_next_cols(jn, M, r + SP_BLOCK_ROWS, n, tile + 1 < tiles.end, t.part);
   finish_first<2>(p, a1, a2, t, sh, red);
   const double rs = p[0], rho_ = p[1];
   int k = (omega == 0.0 || alpha == 0.0) ? -11 : kprev + 1;
@@ -531,7 +558,9 @@ __global__ void __launch_bounds__(SBS) k_kpq(BellView M, const TM* __restrict__ 
       e1 = v.invd[r];
       e2 = rp[r];
     }
-    spmv2_group<TM>(M, vals, rp, r, n, t, red, a1, a2);
+    const SlotCols jc = jn;
+    load_next_cols(jn, M, r + SP_BLOCK_ROWS, n, tile + 1 < tiles.end, t.part);
+    spmv2_group_pre<TM>(M, vals, rp, jc, r, n, t, red, a1, a2);
   }
   put_partials<SBS, 1>({d}, {PS_RQ}, w.partial, w.ps, sh, run);
 }
@@ -563,6 +592,8 @@ __global__ void __launch_bounds__(SBS) k_kst(BellView M, const TM* __restrict__ 
   double p[1] = {0.0};
   for (int i = threadIdx.x; i < w.ps; i += SBS) p[0] += w.partial[PS_RQ * w.ps + i];  // before the SpMV
   if (!first_group<TM>(M, vals, rq, r, n, t, done, a1, a2)) return;
+  SlotCols jn;  // the next tile's columns, in flight during the reductions and the epilogue
+  load_next_cols(jn, M, r + SP_BLOCK_ROWS, n, tile + 1 < tiles.end, t.part);
   finish_first<1>(p, a1, a2, t, sh, red);
   const double alpha = rho / p[0];
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->alpha2[par] = alpha;
@@ -585,7 +616,9 @@ __global__ void __launch_bounds__(SBS) k_kst(BellView M, const TM* __restrict__ 
       e1 = v.q[r];
       e2 = rq[r];
     }
-    spmv2_group<TM>(M, vals, rq, r, n, t, red, a1, a2);
+    const SlotCols jc = jn;
+    load_next_cols(jn, M, r + SP_BLOCK_ROWS, n, tile + 1 < tiles.end, t.part);
+    spmv2_group_pre<TM>(M, vals, rq, jc, r, n, t, red, a1, a2);
   }
   put_partials<SBS, 3>({ts, tt, ss}, {PS_TS, PS_TT, PS_SS}, w.partial, w.ps, sh, run);
 }
