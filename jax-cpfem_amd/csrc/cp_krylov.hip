@@ -177,7 +177,7 @@ __global__ void __launch_bounds__(KBS) k_inv_diag(BellView M, const double* __re
 
 // ---------------------------------------------------------------------------------
 // BiCGSTAB kernels.  Vector kernels: KrylovWork::nb blocks (<= KMAXB) of KBS threads with
-// grid-stride loops.  SpMV kernels: KrylovWork::nbs blocks of SBS threads (below).  Each
+// grid-stride loops.  SpMV kernels: KrylovWork::nbs blocks of SpG<G>::BS threads (below).  Each
 // reducing kernel writes one partial per block into its slot of w.partial; the kernel that
 // consumes the sum adds the slot's ps entries itself, in every block and in the same order,
 // so every block sees the bitwise-identical scalar.  Scalars that carry over to the next
@@ -276,8 +276,9 @@ __global__ void __launch_bounds__(KBS) k_kinit2(KrylovWork w) {
   }
 }
 
-// ---- SpMV of the BiCGSTAB iteration.  A block of SBS = 1024 threads takes 4 groups of 64
-// consecutive rows (one row per lane) and splits each row's node slots over 4 waves (the
+// ---- SpMV of the BiCGSTAB iteration.  A block of 1024 threads takes 4 groups of 64
+// consecutive rows (one row per lane; 1 group of 256 threads on small systems, see SpG) and
+// splits each row's node slots over 4 waves (the
 // wave of part p takes slots p, p+4, ...), so a launch keeps four times as many waves in
 // flight as a thread-per-row SpMV while every wave still reads the column-major values
 // coalesced.  The four partial row sums meet in LDS and the part-0 wave adds them in fixed
@@ -301,11 +302,16 @@ __global__ void __launch_bounds__(KBS) k_kinit2(KrylovWork w) {
 // instead of lines from the whole vector.  Partial sums are stored at the run index.
 constexpr int SP_ROWS = 64;
 constexpr int SP_PARTS = 4;
-constexpr int SP_GROUPS = 4;
-constexpr int SP_BLOCK_ROWS = SP_ROWS * SP_GROUPS;
-constexpr int SBS = SP_ROWS * SP_PARTS * SP_GROUPS;
 constexpr int SP_NS = (27 + SP_PARTS - 1) / SP_PARTS;  // slots per part on 27-slot meshes
-constexpr int SP_RED = (SP_PARTS - 1) * SP_GROUPS * SP_ROWS;
+// Row groups per block: 4 (1024-thread blocks, 256-row tiles) when the vector has at least one
+// such tile per CU, else 1 (256-thread blocks, 64-row tiles) so that small systems still
+// spread over the CUs.
+template <int G>
+struct SpG {
+  static constexpr int BLOCK_ROWS = SP_ROWS * G;        // rows per tile
+  static constexpr int BS = SP_ROWS * SP_PARTS * G;     // threads per block
+  static constexpr int RED = (SP_PARTS - 1) * G * SP_ROWS;  // LDS doubles per accumulator
+};
 
 constexpr int N_XCD = 8;
 __device__ __forceinline__ int xcd_rank(int b, int nb) {
@@ -315,16 +321,17 @@ __device__ __forceinline__ int xcd_rank(int b, int nb) {
 // Tiles [first, end) of run v of nb over n rows (every run is non-empty when nb <= tiles).
 struct TileRun {
   int64_t first, end;
-  __device__ TileRun(int v, int nb, int64_t n) {
-    const int64_t tiles = (n + SP_BLOCK_ROWS - 1) / SP_BLOCK_ROWS;
+  __device__ TileRun(int v, int nb, int64_t n, int block_rows) {
+    const int64_t tiles = (n + block_rows - 1) / block_rows;
     first = (int64_t)v * tiles / nb;
     end = (int64_t)(v + 1) * tiles / nb;
   }
 };
 
+template <int G>
 struct SpmvLane {
   int lane, part, group;
-  __device__ SpmvLane() : lane(threadIdx.x & 63), part((threadIdx.x >> 6) / SP_GROUPS), group((threadIdx.x >> 6) % SP_GROUPS) {}
+  __device__ SpmvLane() : lane(threadIdx.x & 63), part((threadIdx.x >> 6) / G), group((threadIdx.x >> 6) % G) {}
 };
 
 // Column nodes of one part's share of a row on a 27-slot mesh, and the row's slot count
@@ -385,30 +392,32 @@ __device__ __forceinline__ void spmv2_any(const BellView& M, const TM* __restric
   }
 }
 
-// Part sums of (a1, a2) into the part-0 waves; red: 2 * SP_RED doubles.  Leaves a barrier
-// pending: the caller's next __syncthreads() must precede any reuse of red.
-__device__ __forceinline__ void part_sums_write(double a1, double a2, const SpmvLane& t, double* red) {
+// Part sums of (a1, a2) into the part-0 waves; red: 2 * SpG<G>::RED doubles.  Leaves a
+// barrier pending: the caller's next __syncthreads() must precede any reuse of red.
+template <int G>
+__device__ __forceinline__ void part_sums_write(double a1, double a2, const SpmvLane<G>& t, double* red) {
   if (t.part) {
-    const int o = ((t.part - 1) * SP_GROUPS + t.group) * SP_ROWS + t.lane;
+    const int o = ((t.part - 1) * G + t.group) * SP_ROWS + t.lane;
     red[o] = a1;
-    red[SP_RED + o] = a2;
+    red[SpG<G>::RED + o] = a2;
   }
 }
-__device__ __forceinline__ void part_sums_read(double& a1, double& a2, const SpmvLane& t, const double* red) {
+template <int G>
+__device__ __forceinline__ void part_sums_read(double& a1, double& a2, const SpmvLane<G>& t, const double* red) {
   if (t.part == 0) {
 #pragma unroll
     for (int p = 1; p < SP_PARTS; ++p) {
-      const int o = ((p - 1) * SP_GROUPS + t.group) * SP_ROWS + t.lane;
+      const int o = ((p - 1) * G + t.group) * SP_ROWS + t.lane;
       a1 += red[o];
-      a2 += red[SP_RED + o];
+      a2 += red[SpG<G>::RED + o];
     }
   }
 }
 
 // One row group: (A u, A v) for rows r0 + 64 group + lane, complete in the part-0 waves.
-template <typename TM>
+template <typename TM, int G>
 __device__ __forceinline__ void spmv2_group(const BellView& M, const TM* __restrict__ vals, const double2* __restrict__ uv,
-                                            int64_t r, int64_t n, const SpmvLane& t, double* red, double& a1, double& a2) {
+                                            int64_t r, int64_t n, const SpmvLane<G>& t, double* red, double& a1, double& a2) {
   a1 = a2 = 0.0;
   if (r < n) {
     if (M.n_slots == 27) {
@@ -426,9 +435,9 @@ __device__ __forceinline__ void spmv2_group(const BellView& M, const TM* __restr
 }
 
 // spmv2_group with the row's 27-slot column indices already loaded (jc, valid for r < n).
-template <typename TM>
+template <typename TM, int G>
 __device__ __forceinline__ void spmv2_group_pre(const BellView& M, const TM* __restrict__ vals, const double2* __restrict__ uv,
-                                                const SlotCols& jc, int64_t r, int64_t n, const SpmvLane& t, double* red,
+                                                const SlotCols& jc, int64_t r, int64_t n, const SpmvLane<G>& t, double* red,
                                                 double& a1, double& a2) {
   a1 = a2 = 0.0;
   if (r < n) {
@@ -452,14 +461,15 @@ __device__ __forceinline__ void load_next_cols(SlotCols& jn, const BellView& M, 
 
 // Sums of NSL partial slots from registers already loaded (p[j]: this thread's share), with
 // the part sums of the first row group riding on the same barrier.
-template <int NSL>
-__device__ __forceinline__ void finish_first(double (&p)[NSL], double& a1, double& a2, const SpmvLane& t, double* sh,
+template <int NSL, int G>
+__device__ __forceinline__ void finish_first(double (&p)[NSL], double& a1, double& a2, const SpmvLane<G>& t, double* sh,
                                              double* red) {
+  constexpr int NW = SpG<G>::BS / 64;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
   for (int j = 0; j < NSL; ++j) {
     const double s = wave_sum(p[j]);
-    if (lane == 0) sh[j * (SBS / 64) + wid] = s;
+    if (lane == 0) sh[j * NW + wid] = s;
   }
   part_sums_write(a1, a2, t, red);
   __syncthreads();
@@ -467,7 +477,7 @@ __device__ __forceinline__ void finish_first(double (&p)[NSL], double& a1, doubl
   for (int j = 0; j < NSL; ++j) {
     double s = 0.0;
 #pragma unroll
-    for (int i = 0; i < SBS / 64; ++i) s += sh[j * (SBS / 64) + i];
+    for (int i = 0; i < NW; ++i) s += sh[j * NW + i];
     p[j] = s;
   }
   part_sums_read(a1, a2, t, red);
@@ -476,9 +486,9 @@ __device__ __forceinline__ void finish_first(double (&p)[NSL], double& a1, doubl
 
 // First row group of an SpMV kernel: column indices issued before the done test.  Returns
 // false if the solve is finished (uniform over the grid).
-template <typename TM>
+template <typename TM, int G>
 __device__ __forceinline__ bool first_group(const BellView& M, const TM* __restrict__ vals, const double2* __restrict__ uv,
-                                            int64_t r, int64_t n, const SpmvLane& t, int done, double& a1, double& a2) {
+                                            int64_t r, int64_t n, const SpmvLane<G>& t, int done, double& a1, double& a2) {
   a1 = a2 = 0.0;
   const bool s27 = M.n_slots == 27;
   SlotCols jc;
@@ -496,22 +506,23 @@ __device__ __forceinline__ bool first_group(const BellView& M, const TM* __restr
 // Start of iteration j: q = A phat with phat = rh + beta pp, where beta needs <rhat,r> of
 // iteration j-1; also the convergence test of iteration j-1.  Stores phat, q, qh = M q and
 // the partials of <rhat,q>.
-template <typename TM>
-__global__ void __launch_bounds__(SBS) k_kpq(BellView M, const TM* __restrict__ vals, KVec v, KrylovWork w, int par) {
-  __shared__ double sh[2 * (SBS / 64)];
-  __shared__ double red[2 * SP_RED];
+template <typename TM, int G>
+__global__ void __launch_bounds__(SpG<G>::BS) k_kpq(BellView M, const TM* __restrict__ vals, KVec v, KrylovWork w, int par) {
+  constexpr int BS = SpG<G>::BS, BR = SpG<G>::BLOCK_ROWS;
+  __shared__ double sh[2 * (BS / 64)];
+  __shared__ double red[2 * SpG<G>::RED];
   KrylovScalars* sc = w.sc;
   const int pv = par ^ 1;
   const int done = sc->done;
   const double rho = sc->rho2[pv], alpha = sc->alpha2[pv], omega = sc->omega2[pv], atol2 = sc->atol2;
   const int kprev = sc->k2[pv], maxiter = sc->maxiter;
   const double2* __restrict__ rp = v.rp;
-  const SpmvLane t;
+  const SpmvLane<G> t;
   const int64_t n = w.n;
   const int run = xcd_rank(blockIdx.x, gridDim.x);
-  const TileRun tiles(run, gridDim.x, n);
+  const TileRun tiles(run, gridDim.x, n, BR);
   int64_t tile = tiles.first;
-  int64_t r = tile * SP_BLOCK_ROWS + t.group * SP_ROWS + t.lane;
+  int64_t r = tile * BR + t.group * SP_ROWS + t.lane;
   double e0 = 0.0, e1 = 0.0;             // rhat, M of the row
   double2 e2 = make_double2(0.0, 0.0);  // (rh, pp) of the row
   if (t.part == 0 && r < n) {
@@ -521,13 +532,13 @@ __global__ void __launch_bounds__(SBS) k_kpq(BellView M, const TM* __restrict__ 
   }
   double a1, a2;
   double p[2] = {0.0, 0.0};
-  for (int i = threadIdx.x; i < w.ps; i += SBS) {  // issued before the SpMV: independent of it
+  for (int i = threadIdx.x; i < w.ps; i += BS) {  // issued before the SpMV: independent of it
     p[0] += w.partial[PS_RR * w.ps + i];
     p[1] += w.partial[PS_RHR * w.ps + i];
   }
-  if (!first_group<TM>(M, vals, rp, r, n, t, done, a1, a2)) return;
+  if (!first_group<TM, G>(M, vals, rp, r, n, t, done, a1, a2)) return;
   SlotCols jn;  // the next tile's columns, in flight during the reductions and the epilogue
-  load_next_cols(jn, M, r + SP_BLOCK_ROWS, n, tile + 1 < tiles.end, t.part);
+  load_next_cols(jn, M, r + BR, n, tile + 1 < tiles.end, t.part);
   finish_first<2>(p, a1, a2, t, sh, red);
   const double rs = p[0], rho_ = p[1];
   int k = (omega == 0.0 || alpha == 0.0) ? -11 : kprev + 1;
@@ -552,35 +563,36 @@ __global__ void __launch_bounds__(SBS) k_kpq(BellView M, const TM* __restrict__ 
       d += e0 * q;
     }
     if (++tile >= tiles.end) break;  // uniform over the block
-    r += SP_BLOCK_ROWS;
+    r += BR;
     if (t.part == 0 && r < n) {
       e0 = v.rhat[r];
       e1 = v.invd[r];
       e2 = rp[r];
     }
     const SlotCols jc = jn;
-    load_next_cols(jn, M, r + SP_BLOCK_ROWS, n, tile + 1 < tiles.end, t.part);
-    spmv2_group_pre<TM>(M, vals, rp, jc, r, n, t, red, a1, a2);
+    load_next_cols(jn, M, r + BR, n, tile + 1 < tiles.end, t.part);
+    spmv2_group_pre<TM, G>(M, vals, rp, jc, r, n, t, red, a1, a2);
   }
-  put_partials<SBS, 1>({d}, {PS_RQ}, w.partial, w.ps, sh, run);
+  put_partials<BS, 1>({d}, {PS_RQ}, w.partial, w.ps, sh, run);
 }
 
 // t = A shat with shat = rh - alpha qh, alpha = rho / <rhat,q>.  Stores s = r - alpha q, shat,
 // t and the partials of <t,s>, <t,t>, <s,s>.
-template <typename TM>
-__global__ void __launch_bounds__(SBS) k_kst(BellView M, const TM* __restrict__ vals, KVec v, KrylovWork w, int par) {
-  __shared__ double sh[3 * (SBS / 64)];
-  __shared__ double red[2 * SP_RED];
+template <typename TM, int G>
+__global__ void __launch_bounds__(SpG<G>::BS) k_kst(BellView M, const TM* __restrict__ vals, KVec v, KrylovWork w, int par) {
+  constexpr int BS = SpG<G>::BS, BR = SpG<G>::BLOCK_ROWS;
+  __shared__ double sh[3 * (BS / 64)];
+  __shared__ double red[2 * SpG<G>::RED];
   KrylovScalars* sc = w.sc;
   const int done = sc->done;
   const double rho = sc->rho2[par];
   const double2* __restrict__ rq = v.rq;
-  const SpmvLane t;
+  const SpmvLane<G> t;
   const int64_t n = w.n;
   const int run = xcd_rank(blockIdx.x, gridDim.x);
-  const TileRun tiles(run, gridDim.x, n);
+  const TileRun tiles(run, gridDim.x, n, BR);
   int64_t tile = tiles.first;
-  int64_t r = tile * SP_BLOCK_ROWS + t.group * SP_ROWS + t.lane;
+  int64_t r = tile * BR + t.group * SP_ROWS + t.lane;
   double e0 = 0.0, e1 = 0.0;             // r, q of the row
   double2 e2 = make_double2(0.0, 0.0);  // (rh, qh) of the row
   if (t.part == 0 && r < n) {
@@ -590,10 +602,10 @@ __global__ void __launch_bounds__(SBS) k_kst(BellView M, const TM* __restrict__ 
   }
   double a1, a2;
   double p[1] = {0.0};
-  for (int i = threadIdx.x; i < w.ps; i += SBS) p[0] += w.partial[PS_RQ * w.ps + i];  // before the SpMV
-  if (!first_group<TM>(M, vals, rq, r, n, t, done, a1, a2)) return;
+  for (int i = threadIdx.x; i < w.ps; i += BS) p[0] += w.partial[PS_RQ * w.ps + i];  // before the SpMV
+  if (!first_group<TM, G>(M, vals, rq, r, n, t, done, a1, a2)) return;
   SlotCols jn;  // the next tile's columns, in flight during the reductions and the epilogue
-  load_next_cols(jn, M, r + SP_BLOCK_ROWS, n, tile + 1 < tiles.end, t.part);
+  load_next_cols(jn, M, r + BR, n, tile + 1 < tiles.end, t.part);
   finish_first<1>(p, a1, a2, t, sh, red);
   const double alpha = rho / p[0];
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->alpha2[par] = alpha;
@@ -610,17 +622,17 @@ __global__ void __launch_bounds__(SBS) k_kst(BellView M, const TM* __restrict__ 
       ss += s * s;
     }
     if (++tile >= tiles.end) break;  // uniform over the block
-    r += SP_BLOCK_ROWS;
+    r += BR;
     if (t.part == 0 && r < n) {
       e0 = v.r[r];
       e1 = v.q[r];
       e2 = rq[r];
     }
     const SlotCols jc = jn;
-    load_next_cols(jn, M, r + SP_BLOCK_ROWS, n, tile + 1 < tiles.end, t.part);
-    spmv2_group_pre<TM>(M, vals, rq, jc, r, n, t, red, a1, a2);
+    load_next_cols(jn, M, r + BR, n, tile + 1 < tiles.end, t.part);
+    spmv2_group_pre<TM, G>(M, vals, rq, jc, r, n, t, red, a1, a2);
   }
-  put_partials<SBS, 3>({ts, tt, ss}, {PS_TS, PS_TT, PS_SS}, w.partial, w.ps, sh, run);
+  put_partials<BS, 3>({ts, tt, ss}, {PS_TS, PS_TT, PS_SS}, w.partial, w.ps, sh, run);
 }
 
 // omega, x and r updates; stores rh = M r, pp = phat - omega qh and the partials of <r,r>,
@@ -729,11 +741,7 @@ void krylov_init(const BellView& M, const TM* vals, const double* b, KVec v, Kry
 }
 
 int krylov_blocks(int64_t n) { return (int)std::min<int64_t>(nblk(n), KMAXB); }
-// One SpMV block per CU of the current device (a 1024-thread block at this kernel's register
-// use fills a CU, so a larger grid runs in two rounds; fewer tiles than CUs: one per tile).
-// Measured on the 32^3 matrix (422 tiles): 256 blocks on 256 CUs, 46.3 us per BiCGSTAB
-// iteration; 384 blocks 61.7 us; 422 blocks 50.4 us.
-int krylov_spmv_blocks(int64_t n) {
+static int device_cus() {
   static int cus[64] = {0};
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
@@ -742,8 +750,21 @@ int krylov_spmv_blocks(int64_t n) {
     if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0) c = 256;
     cus[dev] = c;
   }
-  const int64_t tiles = (n + SP_BLOCK_ROWS - 1) / SP_BLOCK_ROWS;
-  return (int)std::max<int64_t>(1, std::min<int64_t>(tiles, cus[dev]));
+  return cus[dev];
+}
+
+// Row groups per SpMV block: 4 when there is at least one 256-row tile per CU, else 1.
+int krylov_spmv_groups(int64_t n) { return (n + SpG<4>::BLOCK_ROWS - 1) / SpG<4>::BLOCK_ROWS >= device_cus() ? 4 : 1; }
+
+// SpMV blocks: with 4 groups, one 1024-thread block per CU (at this kernel's register use it
+// fills a CU, so a larger grid runs in two rounds).  Measured on the 32^3 matrix (422 tiles):
+// 256 blocks on 256 CUs, 46.3 us per BiCGSTAB iteration; 384 blocks 61.7 us; 422 blocks
+// 50.4 us.  With 1 group (small systems), one 256-thread block per 64-row tile, at most four
+// per CU.
+int krylov_spmv_blocks(int64_t n) {
+  if (krylov_spmv_groups(n) == 4)
+    return (int)std::max<int64_t>(1, std::min<int64_t>((n + SpG<4>::BLOCK_ROWS - 1) / SpG<4>::BLOCK_ROWS, device_cus()));
+  return (int)std::max<int64_t>(1, std::min<int64_t>((n + SpG<1>::BLOCK_ROWS - 1) / SpG<1>::BLOCK_ROWS, 4 * device_cus()));
 }
 
 void krylov_replace(const BellView& M, const double* vals64, const double* b, KVec v, KrylovWork w, int offer,
@@ -764,8 +785,13 @@ static inline void timed_launch(Prof* prof, int cls, F kernel, dim3 grid, dim3 b
 template <typename TM>
 void krylov_iteration(const BellView& M, const TM* vals, KVec v, KrylovWork w, int par, hipStream_t st, Prof* prof) {
   const unsigned nb = (unsigned)w.nb, nbs = (unsigned)w.nbs;
-  timed_launch(prof, PC_SPMV, k_kpq<TM>, dim3(nbs), dim3(SBS), st, M, vals, v, w, par);
-  timed_launch(prof, PC_SPMV, k_kst<TM>, dim3(nbs), dim3(SBS), st, M, vals, v, w, par);
+  if (w.sg == 4) {
+    timed_launch(prof, PC_SPMV, k_kpq<TM, 4>, dim3(nbs), dim3(SpG<4>::BS), st, M, vals, v, w, par);
+    timed_launch(prof, PC_SPMV, k_kst<TM, 4>, dim3(nbs), dim3(SpG<4>::BS), st, M, vals, v, w, par);
+  } else {
+    timed_launch(prof, PC_SPMV, k_kpq<TM, 1>, dim3(nbs), dim3(SpG<1>::BS), st, M, vals, v, w, par);
+    timed_launch(prof, PC_SPMV, k_kst<TM, 1>, dim3(nbs), dim3(SpG<1>::BS), st, M, vals, v, w, par);
+  }
   timed_launch(prof, PC_KVEC, k_kxr, dim3(nb), dim3(KBS), st, v, w, par);
 }
 
