@@ -53,9 +53,9 @@ def measured_traffic(workload_key, precision, kernels):
         return None
     vals = {}
     for line in open(path):
-        f = line.split()
-        if len(f) == 4 and not line.startswith("#") and f[0] == workload_key and f[1] == precision:
-            vals[f[2]] = float(f[3])
+        f = line.split()  # workload precision kernel-name (may contain spaces) bytes
+        if len(f) >= 4 and not line.startswith("#") and f[0] == workload_key and f[1] == precision:
+            vals[" ".join(f[2:-1])] = float(f[-1])
     got = [vals[k] for k in kernels if k in vals]
     return sum(got) / len(got) if len(got) == len(kernels) else None
 
