@@ -3,7 +3,7 @@
 This package is a plain FP64 NumPy/SciPy restatement of the reference algorithm
 (SuperkakaSCU/JAX-CPFEM, built on the external jax-fem library).  Only ``tests/``,
 ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg may import it,
-and only as the checker / CPU baseline: the product path (``cpfem_amd``) never
+and only as the reference those compare against / the CPU baseline: the product path (``cpfem_amd``) never
 imports, links or executes anything from here, and fails loudly when its HIP
 library is missing.
 

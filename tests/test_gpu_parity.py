@@ -1,6 +1,6 @@
 """HIP path vs the oracle on the same seeded inputs (FP64 parity).
 
-Bars (per BASELINE north star): per-point stress and tangent within 1e-10 relative;
+Bars: per-point stress and tangent within 1e-10 relative;
 assembled residual / Jacobian within 1e-10 relative (in norm); averaged stress within 1e-8."""
 import numpy as np
 import pytest
