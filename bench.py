@@ -249,7 +249,7 @@ def main():
                          "elastic Newton matrix of the FCC polycrystal at --mesh (default 20)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "cpfem_amd runs on one GPU; runs on several GPUs are not supported", "n_gpus": args.gpus}))
         sys.exit(2)
     if args.direct:
         direct_bench(args.mesh or 20, args.steps)
