@@ -322,8 +322,8 @@ def main():
         ach = sb / (t_launch * 1e-3) / 1e9
         tkey = f"poly{args.mesh or 32}" if args.config == 2 else None
         tm = "float" if vb == 4 else "double"
-        # row groups of the SpMV instance the library picks (4 when there is a 256-row tile per CU)
-        sg = 4 if -(-prob.ctx.n_dofs // 256) >= torch.cuda.get_device_properties(0).multi_processor_count else 1
+        # row groups of the SpMV instance the library picks (2 when there are two 128-row tiles per CU)
+        sg = 2 if -(-prob.ctx.n_dofs // 128) >= 2 * torch.cuda.get_device_properties(0).multi_processor_count else 1
         traffic = measured_traffic(tkey, args.precision, [f"cpfem::k_kpq<{tm}, {sg}>",
                                                           f"cpfem::k_kst<{tm}, {sg}>"]) if tkey else None
         roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4),

@@ -303,9 +303,11 @@ __global__ void __launch_bounds__(KBS) k_kinit2(KrylovWork w) {
 constexpr int SP_ROWS = 64;
 constexpr int SP_PARTS = 4;
 constexpr int SP_NS = (27 + SP_PARTS - 1) / SP_PARTS;  // slots per part on 27-slot meshes
-// Row groups per block: 4 (1024-thread blocks, 256-row tiles) when the vector has at least one
-// such tile per CU, else 1 (256-thread blocks, 64-row tiles) so that small systems still
-// spread over the CUs.
+// Row groups per block: 2 (512-thread blocks, 128-row tiles, two blocks per CU) when the
+// vector has at least two such tiles per CU, else 1 (256-thread blocks, 64-row tiles) so that
+// small systems still spread over the CUs.  Two independent blocks per CU keep one block's
+// loads in flight while the other is in its reductions and epilogue (measured against one
+// 1024-thread block per CU: 44.4 vs 45.6 us per iteration at 32^3, 79.8 vs 83.5 us at 40^3).
 template <int G>
 struct SpG {
   static constexpr int BLOCK_ROWS = SP_ROWS * G;        // rows per tile
@@ -753,17 +755,16 @@ static int device_cus() {
   return cus[dev];
 }
 
-// Row groups per SpMV block: 4 when there is at least one 256-row tile per CU, else 1.
-int krylov_spmv_groups(int64_t n) { return (n + SpG<4>::BLOCK_ROWS - 1) / SpG<4>::BLOCK_ROWS >= device_cus() ? 4 : 1; }
+// Row groups per SpMV block: 2 when there are at least two 128-row tiles per CU, else 1.
+int krylov_spmv_groups(int64_t n) { return (n + SpG<2>::BLOCK_ROWS - 1) / SpG<2>::BLOCK_ROWS >= 2 * device_cus() ? 2 : 1; }
 
-// SpMV blocks: with 4 groups, one 1024-thread block per CU (at this kernel's register use it
-// fills a CU, so a larger grid runs in two rounds).  Measured on the 32^3 matrix (422 tiles):
-// 256 blocks on 256 CUs, 46.3 us per BiCGSTAB iteration; 384 blocks 61.7 us; 422 blocks
-// 50.4 us.  With 1 group (small systems), one 256-thread block per 64-row tile, at most four
-// per CU.
+// SpMV blocks: a grid that fits the device in one round, each block walking a run of tiles
+// (at this kernel's register use a CU holds 16 waves: two 512-thread or four 256-thread
+// blocks; a grid beyond that runs in two rounds: on the 32^3 matrix with 1024-thread blocks,
+// 256 blocks took 46.3 us per BiCGSTAB iteration, 384 blocks 61.7 us, 422 blocks 50.4 us).
 int krylov_spmv_blocks(int64_t n) {
-  if (krylov_spmv_groups(n) == 4)
-    return (int)std::max<int64_t>(1, std::min<int64_t>((n + SpG<4>::BLOCK_ROWS - 1) / SpG<4>::BLOCK_ROWS, device_cus()));
+  if (krylov_spmv_groups(n) == 2)
+    return (int)std::max<int64_t>(1, std::min<int64_t>((n + SpG<2>::BLOCK_ROWS - 1) / SpG<2>::BLOCK_ROWS, 2 * device_cus()));
   return (int)std::max<int64_t>(1, std::min<int64_t>((n + SpG<1>::BLOCK_ROWS - 1) / SpG<1>::BLOCK_ROWS, 4 * device_cus()));
 }
 
@@ -785,9 +786,9 @@ static inline void timed_launch(Prof* prof, int cls, F kernel, dim3 grid, dim3 b
 template <typename TM>
 void krylov_iteration(const BellView& M, const TM* vals, KVec v, KrylovWork w, int par, hipStream_t st, Prof* prof) {
   const unsigned nb = (unsigned)w.nb, nbs = (unsigned)w.nbs;
-  if (w.sg == 4) {
-    timed_launch(prof, PC_SPMV, k_kpq<TM, 4>, dim3(nbs), dim3(SpG<4>::BS), st, M, vals, v, w, par);
-    timed_launch(prof, PC_SPMV, k_kst<TM, 4>, dim3(nbs), dim3(SpG<4>::BS), st, M, vals, v, w, par);
+  if (w.sg == 2) {
+    timed_launch(prof, PC_SPMV, k_kpq<TM, 2>, dim3(nbs), dim3(SpG<2>::BS), st, M, vals, v, w, par);
+    timed_launch(prof, PC_SPMV, k_kst<TM, 2>, dim3(nbs), dim3(SpG<2>::BS), st, M, vals, v, w, par);
   } else {
     timed_launch(prof, PC_SPMV, k_kpq<TM, 1>, dim3(nbs), dim3(SpG<1>::BS), st, M, vals, v, w, par);
     timed_launch(prof, PC_SPMV, k_kst<TM, 1>, dim3(nbs), dim3(SpG<1>::BS), st, M, vals, v, w, par);

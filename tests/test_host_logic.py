@@ -98,7 +98,7 @@ def test_measured_traffic_table():
     spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(root, "bench.py"))
     bench = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(bench)
-    t = bench.measured_traffic("poly32", "fp64", ["cpfem::k_kpq<double, 4>", "cpfem::k_kst<double, 4>"])
+    t = bench.measured_traffic("poly32", "fp64", ["cpfem::k_kpq<double, 2>", "cpfem::k_kst<double, 2>"])
     assert t is not None and 60e6 < t < 120e6
     assert bench.measured_traffic("poly32", "fp64", ["cpfem::no_such_kernel"]) is None
-    assert bench.measured_traffic("nowhere", "fp64", ["cpfem::k_kpq<double, 4>"]) is None
+    assert bench.measured_traffic("nowhere", "fp64", ["cpfem::k_kpq<double, 2>"]) is None
