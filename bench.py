@@ -322,10 +322,16 @@ def main():
         ach = sb / (t_launch * 1e-3) / 1e9
         tkey = f"poly{args.mesh or 32}" if args.config == 2 else None
         tm = "float" if vb == 4 else "double"
-        # row groups of the SpMV instance the library picks (2 when there are two 128-row tiles per CU)
-        sg = 2 if -(-prob.ctx.n_dofs // 128) >= 2 * torch.cuda.get_device_properties(0).multi_processor_count else 1
-        traffic = measured_traffic(tkey, args.precision, [f"cpfem::k_kpq<{tm}, {sg}>",
-                                                          f"cpfem::k_kst<{tm}, {sg}>"]) if tkey else None
+        # the SpMV instance the library picks: 128-row tiles when there are two per CU, else
+        # 64-row tiles, with every slot's loads in flight when they fit three blocks per CU
+        cus = torch.cuda.get_device_properties(0).multi_processor_count
+        if -(-prob.ctx.n_dofs // 128) >= 2 * cus:
+            names = [f"cpfem::k_kpq<{tm}, 2>", f"cpfem::k_kst<{tm}, 2>"]
+        elif -(-prob.ctx.n_dofs // 64) <= 3 * cus:
+            names = [f"cpfem::k_kpq_all<{tm}>", f"cpfem::k_kst_all<{tm}>"]
+        else:
+            names = [f"cpfem::k_kpq<{tm}, 1>", f"cpfem::k_kst<{tm}, 1>"]
+        traffic = measured_traffic(tkey, args.precision, names) if tkey else None
         roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4),
                     traffic=(round(traffic) if traffic else None),
                     traffic_source=("rocprofv3 FETCH_SIZE+WRITE_SIZE per dispatch, copy-kernel scaled (profiles/)"

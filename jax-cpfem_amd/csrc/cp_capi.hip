@@ -442,7 +442,7 @@ static int run_bicgstab(cpfem_ctx* c, hipStream_t st, const TM* vals, const doub
   w.n = c->n_dofs;
   w.nb = krylov_blocks(c->n_dofs);
   w.nbs = krylov_spmv_blocks(c->n_dofs);
-  w.sg = krylov_spmv_groups(c->n_dofs);
+  w.sf = krylov_spmv_form(c->n_dofs);
   w.ps = krylov_stride(c->n_dofs);
   w.partial = c->d_kpartial;
   w.sc = c->d_ksc;

@@ -334,6 +334,12 @@ template <int G>
 struct SpmvLane {
   int lane, part, group;
   __device__ SpmvLane() : lane(threadIdx.x & 63), part((threadIdx.x >> 6) / G), group((threadIdx.x >> 6) % G) {}
+  // part and group are wave-uniform: readfirstlane keeps them (and the slot bases derived from
+  // them) in scalar registers, which the all-slots form needs to stay within its VGPR budget
+  __device__ void make_uniform() {
+    part = __builtin_amdgcn_readfirstlane(part);
+    group = __builtin_amdgcn_readfirstlane(group);
+  }
 };
 
 // Column nodes of one part's share of a row on a 27-slot mesh, and the row's slot count
@@ -342,13 +348,27 @@ struct SpmvLane {
 struct SlotCols {
   int32_t j[SP_NS];
   int ns;
-  __device__ __forceinline__ void load(const BellView& M, int64_t r, int part) {
+  // UNIFORM: the slot bases are kept in scalar registers (part is wave-uniform) and the lane
+  // addresses are 32-bit offsets, which saves VGPRs for the all-slots form
+  template <bool UNIFORM = false>
+  __device__ __forceinline__ void load(const BellView& M, int64_t r, int part_) {
     const int64_t i = r / 3;
     ns = M.nslot[i];
+    if constexpr (UNIFORM) {
+      const int part = __builtin_amdgcn_readfirstlane(part_);
+      const uint32_t ioff = (uint32_t)i * (uint32_t)sizeof(int32_t);
 #pragma unroll
-    for (int k = 0; k < SP_NS; ++k) {
-      const int s = part + SP_PARTS * k;
-      j[k] = s < 27 ? M.col[(int64_t)s * M.n_nodes + i] : 0;
+      for (int k = 0; k < SP_NS; ++k) {
+        const int s = part + SP_PARTS * k;
+        const char* cb = reinterpret_cast<const char*>(M.col + (int64_t)(s < 27 ? s : 0) * M.n_nodes);
+        j[k] = s < 27 ? *reinterpret_cast<const int32_t*>(cb + ioff) : 0;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < SP_NS; ++k) {
+        const int s = part_ + SP_PARTS * k;
+        j[k] = s < 27 ? M.col[(int64_t)s * M.n_nodes + i] : 0;
+      }
     }
   }
 };
@@ -373,6 +393,58 @@ __device__ __forceinline__ void spmv2_27(const BellView& M, const TM* __restrict
       }
     }
   }
+}
+
+// spmv2_27 with every slot's loads issued at once, for the 64-row-tile instances.  Those run
+// systems with fewer than two 128-row tiles per CU, where a block walks one or two tiles and
+// a CU holds about one wave per SIMD: the guarded loop's round trip per slot (each guard
+// compiles to an exec-masked block that waits for its loads) is then the kernel's critical
+// path, with nothing to overlap it.  A slot past the row's count reads its own in-bounds
+// padding position (its column is the row's own node) and the slot index past 27 reads the
+// part's first slot; both coefficients are replaced by zero.  Slot bases are wave-uniform and
+// lane addresses 32-bit offsets, which keeps the seven slots in flight at ~165 VGPRs (three
+// waves per SIMD).  Same products, same order as spmv2_27: the sums are bitwise equal.
+template <typename TM>
+__device__ __forceinline__ void spmv2_27_all(const BellView& M, const TM* __restrict__ vals, const double2* __restrict__ uv,
+                                             const SlotCols& jc, int64_t r, int part_, double& a1, double& a2) {
+  const int part = __builtin_amdgcn_readfirstlane(part_);
+  const int64_t nrow = 3 * M.n_nodes;
+  const uint32_t roff = (uint32_t)r * (uint32_t)sizeof(TM);
+  const char* __restrict__ ub = reinterpret_cast<const char*>(uv);
+  double m[SP_NS][3];
+  double2 x[SP_NS][3];
+#pragma unroll
+  for (int k = 0; k < SP_NS; ++k) {
+    const int s = part + SP_PARTS * k;
+    const int su = s < 27 ? s : part;  // wave-uniform
+    const bool ok = s < jc.ns;         // ns <= 27
+    const char* __restrict__ vb = reinterpret_cast<const char*>(vals + 3 * (int64_t)su * nrow);
+    const uint32_t goff = (uint32_t)jc.j[k] * (uint32_t)(3 * sizeof(double2));
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      m[k][c] = (double)*reinterpret_cast<const TM*>(vb + c * nrow * (int64_t)sizeof(TM) + roff);
+      x[k][c] = *reinterpret_cast<const double2*>(ub + goff + c * sizeof(double2));
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) m[k][c] = ok ? m[k][c] : 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < SP_NS; ++k) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      a1 += m[k][c] * x[k][c].x;
+      a2 += m[k][c] * x[k][c].y;
+    }
+  }
+}
+
+template <typename TM, bool ALL>
+__device__ __forceinline__ void spmv2_27g(const BellView& M, const TM* __restrict__ vals, const double2* __restrict__ uv,
+                                          const SlotCols& jc, int64_t r, int part, double& a1, double& a2) {
+  if constexpr (ALL)
+    spmv2_27_all<TM>(M, vals, uv, jc, r, part, a1, a2);
+  else
+    spmv2_27<TM>(M, vals, uv, jc, r, part, a1, a2);
 }
 
 template <typename TM>
@@ -417,15 +489,15 @@ __device__ __forceinline__ void part_sums_read(double& a1, double& a2, const Spm
 }
 
 // One row group: (A u, A v) for rows r0 + 64 group + lane, complete in the part-0 waves.
-template <typename TM, int G>
+template <typename TM, int G, bool ALL>
 __device__ __forceinline__ void spmv2_group(const BellView& M, const TM* __restrict__ vals, const double2* __restrict__ uv,
                                             int64_t r, int64_t n, const SpmvLane<G>& t, double* red, double& a1, double& a2) {
   a1 = a2 = 0.0;
   if (r < n) {
     if (M.n_slots == 27) {
       SlotCols jc;
-      jc.load(M, r, t.part);
-      spmv2_27<TM>(M, vals, uv, jc, r, t.part, a1, a2);
+      jc.load<ALL>(M, r, t.part);
+      spmv2_27g<TM, ALL>(M, vals, uv, jc, r, t.part, a1, a2);
     } else {
       spmv2_any<TM>(M, vals, uv, r, t.part, a1, a2);
     }
@@ -437,14 +509,14 @@ __device__ __forceinline__ void spmv2_group(const BellView& M, const TM* __restr
 }
 
 // spmv2_group with the row's 27-slot column indices already loaded (jc, valid for r < n).
-template <typename TM, int G>
+template <typename TM, int G, bool ALL>
 __device__ __forceinline__ void spmv2_group_pre(const BellView& M, const TM* __restrict__ vals, const double2* __restrict__ uv,
                                                 const SlotCols& jc, int64_t r, int64_t n, const SpmvLane<G>& t, double* red,
                                                 double& a1, double& a2) {
   a1 = a2 = 0.0;
   if (r < n) {
     if (M.n_slots == 27)
-      spmv2_27<TM>(M, vals, uv, jc, r, t.part, a1, a2);
+      spmv2_27g<TM, ALL>(M, vals, uv, jc, r, t.part, a1, a2);
     else
       spmv2_any<TM>(M, vals, uv, r, t.part, a1, a2);
   }
@@ -456,9 +528,10 @@ __device__ __forceinline__ void spmv2_group_pre(const BellView& M, const TM* __r
 
 // Column indices of the row one tile further on, issued early (none past the block's run or
 // past the last row; ns = 0 then).
+template <bool ALL>
 __device__ __forceinline__ void load_next_cols(SlotCols& jn, const BellView& M, int64_t r_next, int64_t n, bool more, int part) {
   jn.ns = 0;
-  if (more && M.n_slots == 27 && r_next < n) jn.load(M, r_next, part);
+  if (more && M.n_slots == 27 && r_next < n) jn.load<ALL>(M, r_next, part);
 }
 
 // Sums of NSL partial slots from registers already loaded (p[j]: this thread's share), with
@@ -488,17 +561,17 @@ __device__ __forceinline__ void finish_first(double (&p)[NSL], double& a1, doubl
 
 // First row group of an SpMV kernel: column indices issued before the done test.  Returns
 // false if the solve is finished (uniform over the grid).
-template <typename TM, int G>
+template <typename TM, int G, bool ALL>
 __device__ __forceinline__ bool first_group(const BellView& M, const TM* __restrict__ vals, const double2* __restrict__ uv,
                                             int64_t r, int64_t n, const SpmvLane<G>& t, int done, double& a1, double& a2) {
   a1 = a2 = 0.0;
   const bool s27 = M.n_slots == 27;
   SlotCols jc;
-  if (s27 && r < n) jc.load(M, r, t.part);
+  if (s27 && r < n) jc.load<ALL>(M, r, t.part);
   if (done) return false;
   if (r < n) {
     if (s27)
-      spmv2_27<TM>(M, vals, uv, jc, r, t.part, a1, a2);
+      spmv2_27g<TM, ALL>(M, vals, uv, jc, r, t.part, a1, a2);
     else
       spmv2_any<TM>(M, vals, uv, r, t.part, a1, a2);
   }
@@ -508,8 +581,9 @@ __device__ __forceinline__ bool first_group(const BellView& M, const TM* __restr
 // Start of iteration j: q = A phat with phat = rh + beta pp, where beta needs <rhat,r> of
 // iteration j-1; also the convergence test of iteration j-1.  Stores phat, q, qh = M q and
 // the partials of <rhat,q>.
-template <typename TM, int G>
-__global__ void __launch_bounds__(SpG<G>::BS) k_kpq(BellView M, const TM* __restrict__ vals, KVec v, KrylovWork w, int par) {
+template <typename TM, int G, bool ALL>
+__device__ __forceinline__ void kpq_body(const BellView& M, const TM* __restrict__ vals, const KVec& v, const KrylovWork& w,
+                                         int par) {
   constexpr int BS = SpG<G>::BS, BR = SpG<G>::BLOCK_ROWS;
   __shared__ double sh[2 * (BS / 64)];
   __shared__ double red[2 * SpG<G>::RED];
@@ -519,7 +593,8 @@ __global__ void __launch_bounds__(SpG<G>::BS) k_kpq(BellView M, const TM* __rest
   const double rho = sc->rho2[pv], alpha = sc->alpha2[pv], omega = sc->omega2[pv], atol2 = sc->atol2;
   const int kprev = sc->k2[pv], maxiter = sc->maxiter;
   const double2* __restrict__ rp = v.rp;
-  const SpmvLane<G> t;
+  SpmvLane<G> t;
+  if constexpr (ALL) t.make_uniform();
   const int64_t n = w.n;
   const int run = xcd_rank(blockIdx.x, gridDim.x);
   const TileRun tiles(run, gridDim.x, n, BR);
@@ -538,9 +613,9 @@ __global__ void __launch_bounds__(SpG<G>::BS) k_kpq(BellView M, const TM* __rest
     p[0] += w.partial[PS_RR * w.ps + i];
     p[1] += w.partial[PS_RHR * w.ps + i];
   }
-  if (!first_group<TM, G>(M, vals, rp, r, n, t, done, a1, a2)) return;
+  if (!first_group<TM, G, ALL>(M, vals, rp, r, n, t, done, a1, a2)) return;
   SlotCols jn;  // the next tile's columns, in flight during the reductions and the epilogue
-  load_next_cols(jn, M, r + BR, n, tile + 1 < tiles.end, t.part);
+  load_next_cols<ALL>(jn, M, r + BR, n, tile + 1 < tiles.end, t.part);
   finish_first<2>(p, a1, a2, t, sh, red);
   const double rs = p[0], rho_ = p[1];
   int k = (omega == 0.0 || alpha == 0.0) ? -11 : kprev + 1;
@@ -572,16 +647,17 @@ __global__ void __launch_bounds__(SpG<G>::BS) k_kpq(BellView M, const TM* __rest
       e2 = rp[r];
     }
     const SlotCols jc = jn;
-    load_next_cols(jn, M, r + BR, n, tile + 1 < tiles.end, t.part);
-    spmv2_group_pre<TM, G>(M, vals, rp, jc, r, n, t, red, a1, a2);
+    load_next_cols<ALL>(jn, M, r + BR, n, tile + 1 < tiles.end, t.part);
+    spmv2_group_pre<TM, G, ALL>(M, vals, rp, jc, r, n, t, red, a1, a2);
   }
   put_partials<BS, 1>({d}, {PS_RQ}, w.partial, w.ps, sh, run);
 }
 
 // t = A shat with shat = rh - alpha qh, alpha = rho / <rhat,q>.  Stores s = r - alpha q, shat,
 // t and the partials of <t,s>, <t,t>, <s,s>.
-template <typename TM, int G>
-__global__ void __launch_bounds__(SpG<G>::BS) k_kst(BellView M, const TM* __restrict__ vals, KVec v, KrylovWork w, int par) {
+template <typename TM, int G, bool ALL>
+__device__ __forceinline__ void kst_body(const BellView& M, const TM* __restrict__ vals, const KVec& v, const KrylovWork& w,
+                                         int par) {
   constexpr int BS = SpG<G>::BS, BR = SpG<G>::BLOCK_ROWS;
   __shared__ double sh[3 * (BS / 64)];
   __shared__ double red[2 * SpG<G>::RED];
@@ -589,7 +665,8 @@ __global__ void __launch_bounds__(SpG<G>::BS) k_kst(BellView M, const TM* __rest
   const int done = sc->done;
   const double rho = sc->rho2[par];
   const double2* __restrict__ rq = v.rq;
-  const SpmvLane<G> t;
+  SpmvLane<G> t;
+  if constexpr (ALL) t.make_uniform();
   const int64_t n = w.n;
   const int run = xcd_rank(blockIdx.x, gridDim.x);
   const TileRun tiles(run, gridDim.x, n, BR);
@@ -605,9 +682,9 @@ __global__ void __launch_bounds__(SpG<G>::BS) k_kst(BellView M, const TM* __rest
   double a1, a2;
   double p[1] = {0.0};
   for (int i = threadIdx.x; i < w.ps; i += BS) p[0] += w.partial[PS_RQ * w.ps + i];  // before the SpMV
-  if (!first_group<TM, G>(M, vals, rq, r, n, t, done, a1, a2)) return;
+  if (!first_group<TM, G, ALL>(M, vals, rq, r, n, t, done, a1, a2)) return;
   SlotCols jn;  // the next tile's columns, in flight during the reductions and the epilogue
-  load_next_cols(jn, M, r + BR, n, tile + 1 < tiles.end, t.part);
+  load_next_cols<ALL>(jn, M, r + BR, n, tile + 1 < tiles.end, t.part);
   finish_first<1>(p, a1, a2, t, sh, red);
   const double alpha = rho / p[0];
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->alpha2[par] = alpha;
@@ -631,10 +708,33 @@ __global__ void __launch_bounds__(SpG<G>::BS) k_kst(BellView M, const TM* __rest
       e2 = rq[r];
     }
     const SlotCols jc = jn;
-    load_next_cols(jn, M, r + BR, n, tile + 1 < tiles.end, t.part);
-    spmv2_group_pre<TM, G>(M, vals, rq, jc, r, n, t, red, a1, a2);
+    load_next_cols<ALL>(jn, M, r + BR, n, tile + 1 < tiles.end, t.part);
+    spmv2_group_pre<TM, G, ALL>(M, vals, rq, jc, r, n, t, red, a1, a2);
   }
   put_partials<BS, 3>({ts, tt, ss}, {PS_TS, PS_TT, PS_SS}, w.partial, w.ps, sh, run);
+}
+
+// The SpMV kernels: 128-row (G = 2) or 64-row (G = 1) tiles with one slot's loads in flight at
+// a time (~110 VGPRs: four waves per SIMD); and 64-row tiles with every slot's loads in flight at once
+// (spmv2_27_all), three waves per SIMD, for systems whose tiles all fit the device at that
+// occupancy (krylov_spmv_form).
+template <typename TM, int G>
+__global__ void __launch_bounds__(SpG<G>::BS) k_kpq(BellView M, const TM* __restrict__ vals, KVec v, KrylovWork w, int par) {
+  kpq_body<TM, G, false>(M, vals, v, w, par);
+}
+template <typename TM, int G>
+__global__ void __launch_bounds__(SpG<G>::BS) k_kst(BellView M, const TM* __restrict__ vals, KVec v, KrylovWork w, int par) {
+  kst_body<TM, G, false>(M, vals, v, w, par);
+}
+template <typename TM>
+__global__ void __launch_bounds__(SpG<1>::BS) __attribute__((amdgpu_waves_per_eu(3, 3)))
+k_kpq_all(BellView M, const TM* __restrict__ vals, KVec v, KrylovWork w, int par) {
+  kpq_body<TM, 1, true>(M, vals, v, w, par);
+}
+template <typename TM>
+__global__ void __launch_bounds__(SpG<1>::BS) __attribute__((amdgpu_waves_per_eu(3, 3)))
+k_kst_all(BellView M, const TM* __restrict__ vals, KVec v, KrylovWork w, int par) {
+  kst_body<TM, 1, true>(M, vals, v, w, par);
 }
 
 // omega, x and r updates; stores rh = M r, pp = phat - omega qh and the partials of <r,r>,
@@ -756,14 +856,22 @@ static int device_cus() {
 }
 
 // Row groups per SpMV block: 2 when there are at least two 128-row tiles per CU, else 1.
-int krylov_spmv_groups(int64_t n) { return (n + SpG<2>::BLOCK_ROWS - 1) / SpG<2>::BLOCK_ROWS >= 2 * device_cus() ? 2 : 1; }
+// SpMV form: 2 = 128-row tiles (at least two per CU), 0 = 64-row tiles with all slot loads in
+// flight (all tiles resident at three 256-thread blocks per CU), 1 = 64-row tiles otherwise.
+int krylov_spmv_form(int64_t n) {
+  const int cus = device_cus();
+  if ((n + SpG<2>::BLOCK_ROWS - 1) / SpG<2>::BLOCK_ROWS >= 2 * cus) return 2;
+  return (n + SpG<1>::BLOCK_ROWS - 1) / SpG<1>::BLOCK_ROWS <= 3 * cus ? 0 : 1;
+}
 
 // SpMV blocks: a grid that fits the device in one round, each block walking a run of tiles
-// (at this kernel's register use a CU holds 16 waves: two 512-thread or four 256-thread
-// blocks; a grid beyond that runs in two rounds: on the 32^3 matrix with 1024-thread blocks,
-// 256 blocks took 46.3 us per BiCGSTAB iteration, 384 blocks 61.7 us, 422 blocks 50.4 us).
+// (at the serial forms' register use a CU holds 16 waves: two 512-thread or four 256-thread
+// blocks; the all-slots form holds three 256-thread blocks and is only picked when every
+// 64-row tile gets a block of its own; a grid beyond that runs in two rounds: on the 32^3
+// matrix with 1024-thread blocks, 256 blocks took 46.3 us per BiCGSTAB iteration, 384 blocks
+// 61.7 us, 422 blocks 50.4 us).
 int krylov_spmv_blocks(int64_t n) {
-  if (krylov_spmv_groups(n) == 2)
+  if (krylov_spmv_form(n) == 2)
     return (int)std::max<int64_t>(1, std::min<int64_t>((n + SpG<2>::BLOCK_ROWS - 1) / SpG<2>::BLOCK_ROWS, 2 * device_cus()));
   return (int)std::max<int64_t>(1, std::min<int64_t>((n + SpG<1>::BLOCK_ROWS - 1) / SpG<1>::BLOCK_ROWS, 4 * device_cus()));
 }
@@ -786,9 +894,12 @@ static inline void timed_launch(Prof* prof, int cls, F kernel, dim3 grid, dim3 b
 template <typename TM>
 void krylov_iteration(const BellView& M, const TM* vals, KVec v, KrylovWork w, int par, hipStream_t st, Prof* prof) {
   const unsigned nb = (unsigned)w.nb, nbs = (unsigned)w.nbs;
-  if (w.sg == 2) {
+  if (w.sf == 2) {
     timed_launch(prof, PC_SPMV, k_kpq<TM, 2>, dim3(nbs), dim3(SpG<2>::BS), st, M, vals, v, w, par);
     timed_launch(prof, PC_SPMV, k_kst<TM, 2>, dim3(nbs), dim3(SpG<2>::BS), st, M, vals, v, w, par);
+  } else if (w.sf == 0) {
+    timed_launch(prof, PC_SPMV, k_kpq_all<TM>, dim3(nbs), dim3(SpG<1>::BS), st, M, vals, v, w, par);
+    timed_launch(prof, PC_SPMV, k_kst_all<TM>, dim3(nbs), dim3(SpG<1>::BS), st, M, vals, v, w, par);
   } else {
     timed_launch(prof, PC_SPMV, k_kpq<TM, 1>, dim3(nbs), dim3(SpG<1>::BS), st, M, vals, v, w, par);
     timed_launch(prof, PC_SPMV, k_kst<TM, 1>, dim3(nbs), dim3(SpG<1>::BS), st, M, vals, v, w, par);

@@ -1,0 +1,66 @@
+"""The three forms of the BiCGSTAB SpMV kernels, each at a size that selects it on a 256-CU
+device: 64-row tiles with every slot's loads in flight (16^3, 14,739 dofs: 231 tiles), 64-row
+tiles one slot at a time (26^3, 59,049 dofs: 923 tiles, above three per CU), 128-row tiles
+(28^3, 73,167 dofs: 572 tiles, two per CU).  The oracle is too slow at these sizes, so each
+case checks what fixes the answer independently of the device kernels, on the first Newton
+system of the 304-like polycrystal (elastic tangent, Dirichlet values in x0), with the true
+residual computed by scipy from the exported CSR.  The recurrence stops at
+|r| <= 1e-10 |b|; the true residual then sits at that level plus the recurrence's rounding
+drift, which grows with the initial residual r0 = b - A x0 (here far larger than b: b is
+nonzero only in the Dirichlet rows, x0 carries their values into the coupled rows), about
+k eps |r0| after k iterations.  Bar: |b - A x| <= 1e-10 |b| + 1e-9 |r0| in FP64 and in the
+mixed form (whose outer loop stops on the FP64 residual); a wrong slot or coefficient leaves
+a residual of the order of |r0|.  The mixed solution lands within 1e-6 of the FP64 one."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def _form(n_dofs, cus):
+    """The form the library picks (krylov_spmv_form): 2, 0 or 1."""
+    if -(-n_dofs // 128) >= 2 * cus:
+        return 2
+    return 0 if -(-n_dofs // 64) <= 3 * cus else 1
+
+
+@pytest.mark.parametrize("n,form", [(16, 0), (26, 1), (28, 2)])
+def test_spmv_form_solves_first_newton_system(n, form):
+    import cpfem_amd as C
+    from cpfem_amd import configs
+    from cpfem_amd.solver import bc_x0, krylov_solve
+    w = configs.config2(n=n)
+    prob = configs.build_problem(w)
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    picked = _form(prob.ctx.n_dofs, cus)
+    print(f"n={n}: {prob.ctx.n_dofs} dofs, {cus} CUs, SpMV form {picked}", flush=True)
+    if cus == 256:
+        assert picked == form
+    info = w.dirichlet_bc_info(w.disps[1])
+    prob.fes[0].update_Dirichlet_boundary_conditions(info)
+    prob.dt = w.ts[1] - w.ts[0]
+    prob.set_params(prob.internal_vars)
+    dofs = torch.zeros(prob.ctx.n_dofs, dtype=torch.float64, device="cuda")
+    res, mat = prob.assemble(dofs)
+    x0 = bc_x0(prob, dofs)
+    b = -res
+    x = krylov_solve(prob, mat, b, x0)
+    it = prob.solver_info[-1]["iterations"]
+    A = C.BlockEllMatrix(prob.ctx, mat).to_scipy()
+    xh, bh, x0h = x.cpu().numpy(), b.cpu().numpy(), x0.cpu().numpy()
+    nb, nr0 = np.linalg.norm(bh), np.linalg.norm(bh - A @ x0h)
+    bar = 1e-10 * nb + 1e-9 * nr0
+    res = np.linalg.norm(A @ xh - bh)
+    print(f"  |r0|/|b| = {nr0 / nb:.2e}; FP64: {it} iterations, |Ax - b|/|b| = {res / nb:.2e}, "
+          f"|Ax - b|/|r0| = {res / nr0:.2e}", flush=True)
+    assert 0 < it < 10000
+    assert res <= bar, (res, bar)
+    xm = krylov_solve(prob, mat, b, x0, dict(precision="mixed")).cpu().numpy()
+    info = prob.solver_info[-1]
+    resm = np.linalg.norm(A @ xm - bh)
+    err = np.linalg.norm(xm - xh) / np.linalg.norm(xh)
+    print(f"  mixed: {info['iterations']} iterations in {info['outer']} passes, |Ax - b|/|r0| = {resm / nr0:.2e}, "
+          f"|x - x64|/|x64| = {err:.2e}", flush=True)
+    assert resm <= bar, (resm, bar)
+    assert err < 1e-6, err
