@@ -290,12 +290,13 @@ __global__ void __launch_bounds__(KBS) k_kinit2(KrylovWork w) {
 // (A (u + c v) = A u + c A v; equal up to rounding), so the matrix stream never waits for the
 // reduction: a kernel loads its scalars and the first row's epilogue operands, issues its
 // column indices before it tests the device-side `done` flag, streams values and gathers,
-// and meanwhile loads the partial sums it reduces.  (Measured on the 32^3 matrix: this form
-// at 1024-thread blocks, 4 waves/SIMD and one block per CU beats both a single-accumulator
-// form that reduces first and smaller blocks at higher occupancy.)
+// and meanwhile loads the partial sums it reduces.  (Measured on the 32^3 matrix with
+// 1024-thread blocks at 4 waves/SIMD: this form beats both a single-accumulator form that
+// reduces first and smaller blocks at higher occupancy.)
 //
-// Work split: the 256-row tiles are dealt out as contiguous runs, one run per block (a block
-// walks its run; the next tile's epilogue operands are loaded before its SpMV starts).  XCD
+// Work split: the row tiles (128 or 64 rows) are dealt out as contiguous runs, one run per
+// block (a block walks its run; the next tile's epilogue operands and column indices are
+// loaded before its SpMV starts).  XCD
 // locality: the hardware hands workgroups to the eight XCDs round-robin (block b runs on XCD
 // b % 8, each XCD with its own L2), so block b takes run xcd_rank(b), which gives every XCD
 // one contiguous range of rows: its L2 then holds one slab of the gathered vector and its halo
@@ -305,7 +306,8 @@ constexpr int SP_PARTS = 4;
 constexpr int SP_NS = (27 + SP_PARTS - 1) / SP_PARTS;  // slots per part on 27-slot meshes
 // Row groups per block: 2 (512-thread blocks, 128-row tiles, two blocks per CU) when the
 // vector has at least two such tiles per CU, else 1 (256-thread blocks, 64-row tiles) so that
-// small systems still spread over the CUs.  Two independent blocks per CU keep one block's
+// small systems still spread over the CUs; there, when every 64-row tile can have a block of
+// its own at three blocks per CU, the all-slots form (spmv2_27_all) runs instead.  Two independent blocks per CU keep one block's
 // loads in flight while the other is in its reductions and epilogue (measured against one
 // 1024-thread block per CU: 44.4 vs 45.6 us per iteration at 32^3, 79.8 vs 83.5 us at 40^3).
 template <int G>
