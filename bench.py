@@ -45,6 +45,18 @@ def spmv_bytes(prob, value_bytes=8):
     return 9 * value_bytes * nnzb + idx + 7 * 8 * n, nnzb
 
 
+def spmv_kernel_names(n_dofs, cus, tm):
+    """Names of the BiCGSTAB SpMV kernel pair the library runs for n_dofs rows on a device
+    with `cus` compute units (the library's krylov_spmv_form): 128-row tiles when there are
+    two per CU, else 64-row tiles, with every slot's loads in flight when all of them fit
+    three blocks per CU."""
+    if -(-n_dofs // 128) >= 2 * cus:
+        return [f"cpfem::k_kpq<{tm}, 2>", f"cpfem::k_kst<{tm}, 2>"]
+    if -(-n_dofs // 64) <= 3 * cus:
+        return [f"cpfem::k_kpq_all<{tm}>", f"cpfem::k_kst_all<{tm}>"]
+    return [f"cpfem::k_kpq<{tm}, 1>", f"cpfem::k_kst<{tm}, 1>"]
+
+
 def measured_traffic(workload_key, precision, kernels):
     """HBM bytes per dispatch of `kernels` (mean) from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary (profiles/traffic_<workload>.txt), or None if it has no entry."""
@@ -322,15 +334,7 @@ def main():
         ach = sb / (t_launch * 1e-3) / 1e9
         tkey = f"poly{args.mesh or 32}" if args.config == 2 else None
         tm = "float" if vb == 4 else "double"
-        # the SpMV instance the library picks: 128-row tiles when there are two per CU, else
-        # 64-row tiles, with every slot's loads in flight when they fit three blocks per CU
-        cus = torch.cuda.get_device_properties(0).multi_processor_count
-        if -(-prob.ctx.n_dofs // 128) >= 2 * cus:
-            names = [f"cpfem::k_kpq<{tm}, 2>", f"cpfem::k_kst<{tm}, 2>"]
-        elif -(-prob.ctx.n_dofs // 64) <= 3 * cus:
-            names = [f"cpfem::k_kpq_all<{tm}>", f"cpfem::k_kst_all<{tm}>"]
-        else:
-            names = [f"cpfem::k_kpq<{tm}, 1>", f"cpfem::k_kst<{tm}, 1>"]
+        names = spmv_kernel_names(prob.ctx.n_dofs, torch.cuda.get_device_properties(0).multi_processor_count, tm)
         traffic = measured_traffic(tkey, args.precision, names) if tkey else None
         roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4),
                     traffic=(round(traffic) if traffic else None),

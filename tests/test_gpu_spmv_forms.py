@@ -19,10 +19,16 @@ torch = pytest.importorskip("torch")
 
 
 def _form(n_dofs, cus):
-    """The form the library picks (krylov_spmv_form): 2, 0 or 1."""
-    if -(-n_dofs // 128) >= 2 * cus:
-        return 2
-    return 0 if -(-n_dofs // 64) <= 3 * cus else 1
+    """The form the library picks (krylov_spmv_form, mirrored by bench.spmv_kernel_names):
+    2 = 128-row tiles, 0 = 64-row tiles with all slot loads in flight, 1 = 64-row tiles."""
+    import importlib.util
+    import os
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(root, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    name = bench.spmv_kernel_names(n_dofs, cus, "double")[0]
+    return 2 if name.endswith(", 2>") else 0 if "_all<" in name else 1
 
 
 @pytest.mark.parametrize("n,form", [(16, 0), (26, 1), (28, 2)])
