@@ -397,11 +397,11 @@ __device__ __forceinline__ void spmv2_27(const BellView& M, const TM* __restrict
   }
 }
 
-// spmv2_27 with every slot's loads issued at once, for the 64-row-tile instances.  Those run
-// systems with fewer than two 128-row tiles per CU, where a block walks one or two tiles and
-// a CU holds about one wave per SIMD: the guarded loop's round trip per slot (each guard
-// compiles to an exec-masked block that waits for its loads) is then the kernel's critical
-// path, with nothing to overlap it.  A slot past the row's count reads its own in-bounds
+// spmv2_27 with every slot's loads issued at once, for small systems: those whose 64-row
+// tiles number at most three per CU, so that every tile has a block of its own and a CU holds
+// about one wave per SIMD.  The guarded loop's round trip per slot (each guard compiles to an
+// exec-masked block that waits for its loads) is then the kernel's critical path, with
+// nothing to overlap it.  (32-bit offsets: such systems have at most 3 x 64 rows per CU.)  A slot past the row's count reads its own in-bounds
 // padding position (its column is the row's own node) and the slot index past 27 reads the
 // part's first slot; both coefficients are replaced by zero.  Slot bases are wave-uniform and
 // lane addresses 32-bit offsets, which keeps the seven slots in flight at ~165 VGPRs (three
