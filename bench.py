@@ -1,21 +1,33 @@
 #!/usr/bin/env python
 """Benchmark of the crystal-plasticity hot path on one MI355X.
 
-A step is one load step of the polycrystal workload (global Newton to the reference
-tolerances: per-point Kalidindi update + tangent, hex8 assembly into the block-ELL Jacobian,
-Jacobi-BiCGSTAB solves; then the cell-averaged Cauchy stress and the history update), run
-through the drop-in ``solver(problem, solver_options)`` loop.  Default workload: 32^3 hex8,
-262,144 quadrature points, 100 Voronoi grains, FCC12, 2% tension in 40 steps of 5e-4.
+Default workload: the dual-phase BCC24 polycrystal of the orientation-design case at 64^3
+hex8 (262,144 cells, 2,097,152 quadrature points, 823,875 dofs), 300 Voronoi grains with
+seeded uniform-SO(3) orientations, 20% of the grains hard (g0 x4, h0 x2), FP64.
+
+A step is one load step (load step 1: 1e-3 strain, dt = 1 s) re-run from its snapshot (the
+state before it), with all of its work: the global Newton to the reference tolerances with
+the dual-phase driver's line search (per-point Kalidindi update + consistent tangent, hex8
+assembly into the block-ELL Jacobian, Jacobi-BiCGSTAB solves, line-search residuals), then
+the cell-averaged Cauchy stress and the history update, through the drop-in
+``solver(problem, solver_options)`` loop.  (The reference algorithm, local Newton from S = 0
+with cut-half backtracking, does not get through load step 2 of this workload from 24^3 up:
+some points' local Newton neither converges nor stays finite, in the oracle as on the device,
+so load step 1 is the load step this workload has at full size.)
 
 value = quadrature points x timed load steps / wall time (whole job, inputs resident in HBM).
-The roofline object refers to the kernel class with the largest total time in the timed
-region, timed with HIP events on its launch stream; the CPU baseline is the oracle timed on
-a bounded sample of the same state on one host core (see the 'sample' string).
+``roofline`` refers to the kernel class with the largest total time in the timed region,
+timed with HIP events on its launch stream; ``detail`` carries the mixed-precision load step,
+the adjoint gradients (this workload's load step w.r.t. its 900 Euler angles, and the
+calibration workload's 50-step history w.r.t. five hardening parameters), and the point
+kernel's FP64 roofline.  ``cpu_baseline`` times the C++/OpenMP restatement of the same
+algorithm on the host's cores on a bounded sample of the same state.
 """
 import argparse
 import ctypes
 import json
 import os
+import platform
 import sys
 import time
 
@@ -27,22 +39,38 @@ for _p in (ROOT, os.path.join(ROOT, "jax-cpfem_amd")):
 HBM_PEAK_GBS = 8000.0    # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector
 
+METRIC = "CP quad-pt updates/s, load-step time, adjoint-grad time (mixed vs FP64)"
+
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+# ------------------------------------------------------------------------------ byte / flop models
 def spmv_bytes(prob, value_bytes=8):
     """Algorithmic HBM bytes of one Krylov SpMV launch (k_kpq / k_kst, equal counts): the
     block values (value_bytes each), one int32 column index per 3x3 block (plus the slot
-    counts, read unless every row is padded to 27 slots; padding is not counted), the two FP64 vectors the
-    operand is formed from (read once each), and five FP64 row vectors (k_kpq: reads rhat and
-    the Jacobi vector, writes phat, q, M q; k_kst: reads r, q, writes s, shat, t)."""
+    counts, read unless every row is padded to 27 slots; padding is not counted), the two FP64
+    vectors the operand is formed from (read once each), and five FP64 row vectors (k_kpq:
+    reads rhat and the Jacobi vector, writes phat, q, M q; k_kst: reads r, q, writes s, shat,
+    t)."""
     nslot, _ = prob.ctx.pattern()
     nnzb = int(nslot.sum())
     n = prob.ctx.n_dofs
     idx = 4 * nnzb + (0 if prob.ctx.n_slots == 27 else 4 * prob.ctx.n_nodes)
     return 9 * value_bytes * nnzb + idx + 7 * 8 * n, nnzb
+
+
+def point_flops(n_slip, n_points, newton_iters, residual_evals):
+    """Algorithmic FP64 flops of the assembly's point kernel over n_points point updates with
+    the measured totals of local Newton iterations and residual evaluations: the reference
+    algorithm's per-unit counts (FMA = 2 flops; div, abs, sign, pow, sqrt = 1 each):
+    F_res(N) = 41 N + 350 per residual evaluation, F_jac(N) = 165 N + 2574 per local Jacobian,
+    F_LU = 648 per 9x9 LU and solve (once per iteration), F_tan = 9300 per consistent tangent,
+    F_elem = 4752 + 144 per point for the element stiffness and residual."""
+    N = n_slip
+    f_res, f_jac, f_lu, f_tan, f_elem = 41 * N + 350, 165 * N + 2574, 648, 9300, 4752 + 144
+    return residual_evals * f_res + newton_iters * (f_jac + f_lu) + n_points * (f_tan + f_elem)
 
 
 def spmv_kernel_names(n_dofs, cus, tm):
@@ -72,102 +100,116 @@ def measured_traffic(workload_key, precision, kernels):
     return sum(got) / len(got) if len(got) == len(kernels) else None
 
 
-def cpu_baseline(prob, w, params, sol, per_step, sample_cells=512, krylov_iters=16):
-    """Oracle timed on a bounded sample of the state at the start of the timed region."""
+# ------------------------------------------------------------------------------ library profile
+class Profile:
+    """Kernel-class timing of libcpfem_hip.so (HIP events on the launch stream around every
+    `sample`-th launch of a class) plus the point kernel's work counters."""
+
+    def __init__(self, prob, sample):
+        from cpfem_amd import _native as N
+        self.N, self.L, self.h, self.sample = N, prob.ctx._lib, prob.ctx.h, sample
+
+    def start(self):
+        from cpfem_amd.problem import stream_handle
+        self.N.check(self.L.cpfem_profile(self.h, 1, self.sample))
+        cnt = (ctypes.c_int64 * 2)()
+        self.N.check(self.L.cpfem_point_counts(self.h, stream_handle(), 1, cnt))
+
+    def read(self):
+        from cpfem_amd.problem import stream_handle
+        n = len(self.N.PROF_CLASSES)
+        ms, la, ti = (ctypes.c_double * n)(), (ctypes.c_int64 * n)(), (ctypes.c_int64 * n)()
+        self.N.check(self.L.cpfem_profile_read(self.h, n, ms, la, ti))
+        self.N.check(self.L.cpfem_profile(self.h, 0, 1))
+        cnt = (ctypes.c_int64 * 2)()
+        self.N.check(self.L.cpfem_point_counts(self.h, stream_handle(), 1, cnt))
+        prof = {c: dict(launches=int(la[k]), timed=int(ti[k]), avg_ms=(ms[k] / ti[k] if ti[k] else None),
+                        est_total_ms=(ms[k] / ti[k] * la[k] if ti[k] else 0.0))
+                for k, c in enumerate(self.N.PROF_CLASSES)}
+        return prof, dict(newton_iters=int(cnt[0]), residual_evals=int(cnt[1]))
+
+
+# ------------------------------------------------------------------------------ load steps
+class StepRunner:
+    """Re-runs load step `index` (0-based) of a workload from a fixed snapshot."""
+
+    def __init__(self, prob, w, params, sol, index):
+        self.prob, self.w, self.index = prob, w, index
+        self.params, self.sol = params, sol
+
+    def run(self, precision, on_step=None):
+        from cpfem_amd import configs
+        return configs.run_steps(self.prob, self.w, self.params, [self.sol[0]], self.index, self.index + 1,
+                                 {"precision": precision}, on_step=on_step)
+
+
+# ------------------------------------------------------------------------------ adjoints
+def step_adjoint(prob, w, index, szz_ref, precision):
+    """Gradient of (mean szz after load step `index` - target)^2 w.r.t. the per-grain z-y-x
+    Euler angles (the orientation-design loss of poly_sensitivity_rot_mesh2.py:179-239, with
+    R = R_euler(theta) R_quat per grain as at :203), through the forward load step and its
+    hand-written adjoint (cpfem_amd.simulate).  theta = 0 (the workload's own orientations);
+    target = 1.05 x the FP64 forward value.  Returns timing and the gradient."""
     import numpy as np
     import torch
-    from threadpoolctl import threadpool_limits
-    from oracle import cp_local, fe
-
-    mat = w.material
-    om = cp_local.Material(cp_local.schmid_tensors(__import__("cpfem_amd").materials.load_slip_table(mat.slip)),
-                           xm=mat.xm, gss_a=mat.gss_a, h=mat.h, t_sat=mat.t_sat, r=mat.r, C11=mat.C11, C12=mat.C12,
-                           C44=mat.C44, self_block=mat.self_block)
-    S = sample_cells
-    cells = w.mesh.cells[:S]
-    u = sol[0].reshape(-1).cpu().numpy()
-    sg, jxw = prob.fes[0].shape_grads[:S], prob.fes[0].JxW[:S]
-    ug = np.einsum("cai,cqaj->cqij", u.reshape(-1, 3)[cells], sg).reshape(-1, 3, 3)
-    take = lambda t, comps: t[:S].reshape((S * 8,) + comps).cpu().numpy()
-    Ns = prob.num_slip_sys
-    Fp, g, sl, R = take(params[0], (3, 3)), take(params[1], (Ns,)), take(params[2], (Ns,)), take(params[3], (3, 3))
-    dt = float(w.ts[1] - w.ts[0])
-    with threadpool_limits(1):
-        t0 = time.perf_counter()
-        out = cp_local.local_update(ug, Fp, g, sl, R, om, dt, max_sub_step=mat.max_sub_step)
-        A = out["A"].reshape(S, 8, 3, 3, 3, 3)
-        P = out["P"].reshape(S, 8, 3, 3)
-        np.einsum("cqaj,cqijkl,cqbl,cq->caibk", sg, A, sg, jxw, optimize=True)
-        np.einsum("cqij,cqaj,cq->cai", P, sg, jxw)
-        t_asm = (time.perf_counter() - t0) / (S * 8)
-        t0 = time.perf_counter()
-        cp_local.local_update(ug, Fp, g, sl, R, om, dt, max_sub_step=mat.max_sub_step, want_tangent=False)
-        t_post = (time.perf_counter() - t0) / (S * 8)
-        prob.set_params(params)  # the snapshot history that belongs to `sol`
-        _, mat_vals = prob.assemble(sol[0].reshape(-1))
-        Asp = __import__("cpfem_amd").BlockEllMatrix(prob.ctx, mat_vals).to_scipy()
-        if not np.isfinite(Asp.data).all():
-            raise RuntimeError("CPU baseline: the sampled Jacobian is not finite")
-        # seeded right-hand side far from convergence, so that all krylov_iters iterations run
-        b = Asp @ np.random.default_rng(0).normal(size=Asp.shape[0])
-        inv_d = 1.0 / Asp.diagonal()
-        t0 = time.perf_counter()
-        _, k_run = fe.bicgstab_jax(Asp, b, np.zeros_like(b), M=lambda v: v * inv_d, maxiter=krylov_iters,
-                                   tol=0.0, atol=0.0)
-        t_iter = (time.perf_counter() - t0) / max(int(k_run), 1)
-        if int(k_run) != krylov_iters:
-            raise RuntimeError(f"CPU baseline BiCGSTAB ran {k_run} of {krylov_iters} iterations")
-    Q = w.n_qp
-    step_s = (per_step["assemblies"] * Q * t_asm + per_step["krylov_iters"] * t_iter + 2 * Q * t_post)
-    return dict(value=Q / step_s, unit="quad-pt load-step updates/s", cores=1, kind="port",
-                sample=(f"oracle (NumPy/SciPy FP64, 1 thread) on {S * 8} points of the timed state "
-                        f"({t_asm * 1e6:.1f} us/point update+tangent+element, {t_post * 1e6:.1f} us/point post) and "
-                        f"{krylov_iters} BiCGSTAB iterations on the full {Asp.shape[0]}-dof Jacobian "
-                        f"({t_iter * 1e3:.1f} ms/iter), scaled by the GPU run's per-step counts "
-                        f"({per_step['assemblies']:.1f} assemblies, {per_step['krylov_iters']:.0f} Krylov iterations); "
-                        f"est. {step_s:.1f} s per load step"))
+    import cpfem_amd as C
+    from cpfem_amd import configs
+    from cpfem_amd.rotations import alpha_rotate_tensor_rank_2, get_rot_mat, rotate_theta_transfer
+    if index != 0:
+        raise ValueError("the load-step adjoint starts from the initial state")
+    nc = len(w.mesh.cells)
+    g0, mat8 = configs.history_inputs(w, prob)
+    Rq = torch.as_tensor(get_rot_mat(w.quat), device="cuda")
+    gid = torch.as_tensor(np.clip(w.cell_ori_inds, 0, len(w.quat) - 1), device="cuda")
+    load = C.LoadHistory(w.disps[:2], w.ts[:2], w.dirichlet_bc_info)
+    theta = torch.zeros(len(w.quat), 3, dtype=torch.float64, device="cuda", requires_grad=True)
+    target = 1.05 * szz_ref
+    opts = dict(w.solver_options, precision=precision)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    R = alpha_rotate_tensor_rank_2(rotate_theta_transfer(theta), Rq)[gid].unsqueeze(1).expand(nc, 8, 3, 3)
+    szz = C.simulate(prob, load, g0, mat8, R, checkpoint_every=5, solver_options=opts,
+                     adjoint_solver_options={"precision": precision})[-1, 2, 2]
+    loss = (szz - target) ** 2
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    return dict(forward_s=t1 - t0, backward_s=t2 - t1, grad_s=t2 - t0, szz=float(szz), loss=float(loss),
+                grad=theta.grad.detach().cpu().numpy())
 
 
-def adjoint_bench(n=16, steps=None, k=10, fd=True, mixed=True):
-    """Gradient of a calibration loss w.r.t. five hardening scales through the whole load
-    history (hand-written adjoint, history checkpointed every k steps).  Workload: the
+def history_adjoint(n=16, k=10, fd=True, mixed=True):
+    """Gradient of the calibration loss w.r.t. the five hardening scales through the whole
+    load history (hand-written adjoint, history checkpointed every k steps).  Workload: the
     calibration case (64 grains, 304-like stiffness, hidden (g0, h0, gs, a, m) =
     (90, 450, 280, 2.2, 0.02), 5% in 50 steps, the steel calibration driver's solver options),
     loss = sum over steps of (szz - target)^2 with the target from the hidden parameters and
-    the query = hidden x (1 + U(-0.2, 0.2)) (seeded).  Default mesh 16^3: at 24^3 the forward
-    solve of the hidden parameters itself fails in load step 2 (a point's local Newton
-    overflows), so no gradient exists there.  Also reports a central difference (relative step
-    1e-6) of the g0 component and the gradient of the mixed-precision forward and adjoint."""
+    the query = hidden x (1 + U(-0.2, 0.2)) (seeded).  Default mesh 16^3: from 20^3 up the
+    forward run of the hidden parameters itself fails (a point's local Newton overflows, in the
+    oracle as on the device), so no gradient exists there.  Also reports a central difference
+    (relative step 1e-6) of the g0 component and the gradient of the mixed-precision forward and
+    adjoint."""
     import numpy as np
     import torch
     import cpfem_amd as C
     from cpfem_amd import configs
     w = configs.config4(n=n)
-    if steps is not None:
-        w.disps, w.ts = w.disps[:steps + 1], w.ts[:steps + 1]
     prob = configs.build_problem(w)
-    nc, Ns, m = len(w.mesh.cells), prob.num_slip_sys, w.material
     load = C.LoadHistory(w.disps, w.ts, w.dirichlet_bc_info)
-    R = torch.as_tensor(C.get_rot_mat(w.quat)[w.cell_ori_inds], device="cuda")[:, None].expand(nc, 8, 3, 3)
-
-    def inputs(theta):  # theta: scales of (g0, h0, g_sat, a, m)
-        base = torch.tensor([m.xm, m.gss_a, m.h, m.t_sat, m.r, m.C11, m.C12, m.C44], dtype=torch.float64, device="cuda")
-        one = torch.ones((), dtype=torch.float64, device="cuda")
-        sc = torch.stack([theta[4], theta[3], theta[1], theta[2], one, one, one, one])
-        return (m.g0 * theta[0]).expand(nc, 8, Ns), (base * sc).expand(nc, 8, 8)
-
+    R = configs.history_rotations(w)
     opts = dict(w.solver_options)
 
     def loss_of(theta, target, precision="fp64"):
-        g0, m8 = inputs(theta)
+        g0, m8 = configs.calibration_inputs(w, prob, theta)
         o = dict(opts, precision=precision)
         szz = C.simulate(prob, load, g0, m8, R, checkpoint_every=k, solver_options=o,
                          adjoint_solver_options={"precision": precision})[:, 2, 2]
         return ((szz - target) ** 2).sum() if target is not None else szz
 
     hidden = torch.ones(5, dtype=torch.float64, device="cuda")
-    rng = np.random.default_rng(0)
-    q0 = torch.as_tensor(1.0 + rng.uniform(-0.2, 0.2, size=5), device="cuda")
+    q0 = torch.as_tensor(configs.calibration_query(), device="cuda")
     with torch.no_grad():
         target = loss_of(hidden, None)
     query = q0.clone().requires_grad_(True)
@@ -201,8 +243,48 @@ def adjoint_bench(n=16, steps=None, k=10, fd=True, mixed=True):
         gx = qx.grad.cpu().numpy()
         g64 = np.array(out["grad"])
         out["mixed"] = dict(grad_s=time.perf_counter() - t3, loss_rel_err=abs(float(lx) - out["loss"]) / abs(out["loss"]),
-                            grad_max_rel_err=float(np.max(np.abs(gx - g64)) / np.max(np.abs(g64))))
+                            grad_max_rel_err=float(np.max(np.abs(gx - g64)) / np.max(np.abs(g64))),
+                            speedup_over_fp64=out["grad_s"] / (time.perf_counter() - t3))
     return out
+
+
+# ------------------------------------------------------------------------------ CPU baseline
+def cpu_baseline(prob, w, params, sol, index, per_step, krylov_iters=40):
+    """The C++/OpenMP restatement of the reference algorithm (oracle/cpu) on all host cores,
+    on a bounded sample of the timed load step: one full assembly (every point's local Newton
+    and tangent, element stiffness, CSR gather) at the converged state and `krylov_iters`
+    Jacobi-BiCGSTAB iterations on the assembled CSR matrix, scaled by the GPU run's per-step
+    counts of assemblies (plus one point pass each for the cell stress and the history
+    update) and Krylov iterations."""
+    import numpy as np
+    from oracle.cpu import cpu_ref
+    import torch
+    cores = cpu_ref.threads()
+    mat = cpu_ref.Model.from_problem(prob, params, w)
+    u = sol[0].reshape(-1).detach().cpu().numpy()
+    # the timed step's Dirichlet values (the history is the snapshot's)
+    prob.dt = float(w.ts[index + 1] - w.ts[index])
+    t_asm, t_post, t_it, info = mat.time_sample(u, float(prob.dt), krylov_iters)
+    step_s = (per_step["assemblies"] * t_asm + 2 * t_post + per_step["krylov_iters"] * t_it)
+    Q = w.n_qp
+    model = platform.processor() or platform.machine()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return dict(value=Q / step_s, unit="quad-pt load-step updates/s", cores=cores, kind="port",
+                sample=(f"C++/OpenMP restatement of the reference algorithm (oracle/cpu), {cores} threads on "
+                        f"{os.cpu_count()} logical CPUs ({model}): one full assembly of all {Q} points at the timed "
+                        f"step's converged state ({t_asm:.2f} s: local Newton + tangent + element stiffness + CSR "
+                        f"gather; {info['newton_iters'] / Q:.2f} local iterations per point), one point pass without "
+                        f"tangent ({t_post:.2f} s) and {krylov_iters} Jacobi-BiCGSTAB iterations on the "
+                        f"{info['n_dofs']}-dof CSR matrix ({t_it * 1e3:.1f} ms/iteration), scaled by the GPU run's "
+                        f"per-step counts ({per_step['assemblies']:.1f} assemblies, 2 point passes, "
+                        f"{per_step['krylov_iters']:.0f} Krylov iterations): est. {step_s:.1f} s per load step"),
+                load_step_s=step_s)
 
 
 def direct_bench(n, reps):
@@ -241,21 +323,23 @@ def direct_bench(n, reps):
         "diff_vs_krylov": float(torch.linalg.norm(x - xk) / torch.linalg.norm(x))}))
 
 
+# ------------------------------------------------------------------------------ main
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", type=int, default=2)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", type=int, default=5)
     ap.add_argument("--mesh", type=int, default=None, help="override the mesh size n (n^3 cells)")
+    ap.add_argument("--load-step", type=int, default=1, help="the load step (1-based) every timed step re-runs")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--profile-sample", type=int, default=64,
-                    help="time every n-th launch of each kernel class with HIP events (events perturb the run)")
+                    help="time every n-th launch of each kernel class with HIP events")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--compare-mixed", action="store_true", help="rerun the timed steps in the other precision")
+    ap.add_argument("--mixed-steps", type=int, default=2, help="timed load steps in the other precision (0: none)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip both adjoint gradients")
+    ap.add_argument("--history-adjoint-mesh", type=int, default=16)
     ap.add_argument("--copy-calibration", action="store_true", help="also run a plain copy kernel (counter scale)")
-    ap.add_argument("--adjoint", action="store_true", help="also time the history gradient of the calibration workload")
-    ap.add_argument("--adjoint-steps", type=int, default=None, help="load steps of the adjoint workload (default: all 50)")
     ap.add_argument("--direct", action="store_true",
                     help="instead of the load steps: time the banded direct solve ('umfpack_solver') on the "
                          "elastic Newton matrix of the FCC polycrystal at --mesh (default 20)")
@@ -269,150 +353,181 @@ def main():
 
     import numpy as np
     import torch
-    import cpfem_amd as C
     from cpfem_amd import _native as N
     from cpfem_amd import configs
 
     torch.cuda.set_device(0)
     kw = {} if args.mesh is None else {"n": args.mesh}
     w = configs.CONFIGS[args.config](**kw)
-    log(f"[bench] workload {w.name}: {len(w.mesh.cells)} cells, {w.n_qp} points")
+    n_mesh = round(len(w.mesh.cells) ** (1 / 3))
+    S = args.load_step - 1
+    if not 0 <= S < len(w.ts) - 1:
+        raise SystemExit(f"--load-step must be in 1..{len(w.ts) - 1}")
+    log(f"[bench] workload {w.name}: {len(w.mesh.cells)} cells, {w.n_qp} points; load step {S + 1}")
+    t0 = time.perf_counter()
     prob = configs.build_problem(w)
-    L, h = prob.ctx._lib, prob.ctx.h
+    log(f"[bench] context built in {time.perf_counter() - t0:.1f}s")
     params = prob.internal_vars
     sol = [torch.zeros(prob.fes[0].num_total_nodes, 3, dtype=torch.float64, device="cuda")]
-    opts = {"precision": args.precision}
+    if S > 0:
+        params, sol, _ = configs.run_steps(prob, w, params, sol, 0, S, {"precision": args.precision})
+    runner = StepRunner(prob, w, [p.clone() for p in params], [sol[0].clone()], S)
     W, K = args.warmup, args.steps
-    if W + K > len(w.ts) - 1:
-        raise SystemExit(f"warmup + steps exceeds the {len(w.ts) - 1} load steps of the workload")
-    t0 = time.perf_counter()
-    params, sol, _ = configs.run_steps(prob, w, params, sol, 0, W, opts,
-                                       on_step=lambda i, p, s: log(f"[bench] warmup step {i + 1}: szz={s[2, 2]:.6f}"))
-    log(f"[bench] warmup {W} steps {time.perf_counter() - t0:.2f}s")
-    snap = ([p.clone() for p in params], [sol[0].clone()])
 
-    counts = dict(newton=0, krylov=0, solves=0)
+    t0 = time.perf_counter()
+    for i in range(W):
+        _, _, sig = runner.run(args.precision)
+        log(f"[bench] warmup {i + 1}/{W}: szz={sig[0][2, 2]:.10f} newton={prob.last_solve['newton']}")
+    log(f"[bench] warmup {W} steps {time.perf_counter() - t0:.2f}s")
+
+    counts = dict(newton=0)
 
     def on_step(i, p, s):
         counts["newton"] += p.last_solve["newton"]
-        log(f"[bench] step {i + 1}: szz={s[2, 2]:.6f} newton={p.last_solve['newton']} "
-            f"krylov={[d['iterations'] for d in p.solver_info[-p.last_solve['newton']:]] if p.last_solve['newton'] else []}")
 
+    prof = Profile(prob, args.profile_sample)
     prob.solver_info.clear()
-    N.check(L.cpfem_profile(h, 1, args.profile_sample))
     torch.cuda.synchronize()
+    prof.start()
     t0 = time.perf_counter()
-    params, sol, sig = configs.run_steps(prob, w, params, sol, W, W + K, opts, on_step=on_step)
+    sigs = []
+    for i in range(K):
+        final_params, final_sol, sig = runner.run(args.precision, on_step=on_step)
+        sigs.append(sig[0])
     torch.cuda.synchronize()
     T = time.perf_counter() - t0
-    ms = (ctypes.c_double * 8)()
-    la = (ctypes.c_int64 * 8)()
-    ti = (ctypes.c_int64 * 8)()
-    N.check(L.cpfem_profile_read(h, 8, ms, la, ti))
-    N.check(L.cpfem_profile(h, 0, 1))
+    kp, work = prof.read()
     kry = [d["iterations"] for d in prob.solver_info]
+    log(f"[bench] timed {K} steps in {T:.3f}s ({T / K:.3f} s/step); newton {counts['newton'] / K:.1f}/step; "
+        "profile: " + ", ".join(f"{c}={kp[c]['est_total_ms']:.1f}ms/{kp[c]['launches']}" for c in kp))
     assemblies = counts["newton"] + K  # one assembly per Newton iteration plus the initial one
     per_step = dict(assemblies=assemblies / K, krylov_iters=sum(max(k, 0) for k in kry) / K)
-    classes = N.PROF_CLASSES
-    prof = {c: dict(launches=int(la[k]), timed=int(ti[k]), avg_ms=(ms[k] / ti[k] if ti[k] else None),
-                    est_total_ms=(ms[k] / ti[k] * la[k] if ti[k] else 0.0)) for k, c in enumerate(classes)}
-    dom = max(prof, key=lambda c: prof[c]["est_total_ms"])
-    log(f"[bench] timed {K} steps in {T:.3f}s; profile: " +
-        ", ".join(f"{c}={prof[c]['est_total_ms']:.1f}ms/{prof[c]['launches']}" for c in classes))
 
     Q = w.n_qp
     value = Q * K / T
     vb = 4 if args.precision == "mixed" else 8
     sb, nnzb = spmv_bytes(prob, value_bytes=vb)
-    roof = None
-    if dom == "krylov_spmv" and prof[dom]["avg_ms"]:
-        # Launches that move the matrix: two per BiCGSTAB iteration plus the initial residual of
-        # each solve; the launches queued past convergence return at once and move nothing, so
-        # the class time (sampled mean x all launches) is divided by the working launches only.
-        working = 2 * sum(max(k, 0) for k in kry) + len(kry)
-        t_launch = prof[dom]["est_total_ms"] / working
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    tm = "float" if vb == 4 else "double"
+    names = spmv_kernel_names(prob.ctx.n_dofs, cus, tm)
+    tkey = f"c{args.config}_{n_mesh}"
+    dom = max(("krylov_spmv", "element_qp", "element_K", "gather_matrix", "krylov_vector", "point_local"),
+              key=lambda c: kp[c]["est_total_ms"])
+    # Launches that move the matrix: two per BiCGSTAB iteration plus the initial residual of each
+    # solve (the launches queued past convergence return at once and move nothing, so the class
+    # time, sampled mean x all launches, is divided by the working launches only)
+    working = 2 * sum(max(k, 0) for k in kry) + len(kry)
+    spmv_roof = None
+    if kp["krylov_spmv"]["avg_ms"] and working:
+        t_launch = kp["krylov_spmv"]["est_total_ms"] / working
         ach = sb / (t_launch * 1e-3) / 1e9
-        tkey = f"poly{args.mesh or 32}" if args.config == 2 else None
-        tm = "float" if vb == 4 else "double"
-        names = spmv_kernel_names(prob.ctx.n_dofs, torch.cuda.get_device_properties(0).multi_processor_count, tm)
-        traffic = measured_traffic(tkey, args.precision, names) if tkey else None
-        roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4),
-                    traffic=(round(traffic) if traffic else None),
-                    traffic_source=("rocprofv3 FETCH_SIZE+WRITE_SIZE per dispatch, copy-kernel scaled (profiles/)"
-                                    if traffic else None),
-                    kernel="k_kpq/k_kst (BiCGSTAB SpMV with fused p/s updates and dots)",
-                    bytes_per_launch=sb, avg_launch_us=round(t_launch * 1e3, 2), working_launches=working,
-                    timing="hipExtLaunchKernelGGL start/stop events (kernel-bound) on every "
-                           f"{args.profile_sample}th launch of the class, on its launch stream")
-    elif prof[dom]["avg_ms"]:
-        # another class dominates (small meshes: the FP64-VALU point kernels); no byte model here
-        roof = dict(bound=None, achieved=None, peak=None, unit=None, frac=None, traffic=None,
-                    kernel=dom, avg_launch_us=round(prof[dom]["avg_ms"] * 1e3, 2))
-
+        traffic = measured_traffic(tkey, args.precision, names)
+        spmv_roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                         frac=round(ach / HBM_PEAK_GBS, 4), traffic=(round(traffic) if traffic else None),
+                         traffic_source=(f"rocprofv3 FETCH_SIZE+WRITE_SIZE per dispatch, copy-kernel scaled "
+                                         f"(profiles/traffic_{tkey}.txt)" if traffic else None),
+                         kernel=f"{names[0]} / {names[1]} (BiCGSTAB SpMV with fused p/s updates and dots)",
+                         bytes_per_launch=sb, avg_launch_us=round(t_launch * 1e3, 2), working_launches=working,
+                         timing="hipExtLaunchKernelGGL start/stop events (kernel-bound) on every "
+                                f"{args.profile_sample}th launch of the class, on its launch stream")
+    point_roof = None
+    qp = kp["element_qp"]
+    if qp["avg_ms"] and qp["launches"]:
+        flops = point_flops(prob.num_slip_sys, Q * qp["launches"], work["newton_iters"], work["residual_evals"])
+        ach_tf = flops / (qp["est_total_ms"] * 1e-3) / 1e12
+        point_roof = dict(bound="fp64 valu", achieved=round(ach_tf, 3), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
+                          frac=round(ach_tf / FP64_PEAK_TFLOPS, 4),
+                          kernel=f"cpfem::k_element_qp<{prob.num_slip_sys}> (local Newton + tangent per point)",
+                          launches=qp["launches"], avg_launch_ms=round(qp["avg_ms"], 4),
+                          flops_per_point=round(flops / (Q * qp["launches"]), 1),
+                          newton_iters_per_point=round(work["newton_iters"] / (Q * qp["launches"]), 3),
+                          residual_evals_per_point=round(work["residual_evals"] / (Q * qp["launches"]), 3),
+                          flop_model="F_res(N)=41N+350 per residual evaluation, F_jac(N)=165N+2574 + F_LU=648 per "
+                                     "iteration, F_tan=9300 + F_elem=4896 per point (reference algorithm counts)")
+    roof = spmv_roof if dom == "krylov_spmv" else (point_roof if dom == "element_qp" else
+                                                    dict(bound=None, achieved=None, peak=None, unit=None, frac=None,
+                                                         traffic=None, kernel=dom))
+    szz = [float(s[2, 2]) for s in sigs]
     detail = dict(
-        kernel_profile=prof, dominant=dom, newton_per_step=counts["newton"] / K, krylov_per_solve=kry,
-        point_evals_per_s_in_update_kernel=(Q * prof["element_qp"]["launches"] / (prof["element_qp"]["est_total_ms"] * 1e-3)
-                                            if prof["element_qp"]["est_total_ms"] else None),
-        sigma_bar_zz=[float(s[2, 2]) for s in sig], nnz_blocks=nnzb, n_dofs=prob.ctx.n_dofs,
-        spmv_bytes_per_launch=sb, load_step_s=T / K)
+        kernel_profile=kp, dominant=dom, newton_per_step=counts["newton"] / K, krylov_per_solve=kry[: 40],
+        per_step=per_step, point_work=work, sigma_bar_zz=szz[0], sigma_bar_zz_spread=max(szz) - min(szz),
+        nnz_blocks=nnzb, n_dofs=prob.ctx.n_dofs, spmv_bytes_per_launch=sb, load_step_s=T / K,
+        spmv_roofline=spmv_roof, point_roofline=point_roof)
 
-    if args.compare_mixed:
+    if args.mixed_steps > 0:
         other = "mixed" if args.precision == "fp64" else "fp64"
-        p2, s2 = snap
+        runner.run(other)  # warm-up (the FP32 copy of the matrix is allocated on first use)
         prob.solver_info.clear()
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        _, _, sig2 = configs.run_steps(prob, w, p2, s2, W, W + K, {"precision": other})
+        for _ in range(args.mixed_steps):
+            _, _, sig2 = runner.run(other)
         torch.cuda.synchronize()
-        T2 = time.perf_counter() - t1
-        ref = np.array([s[2, 2] for s in (sig if args.precision == "fp64" else sig2)])
-        mix = np.array([s[2, 2] for s in (sig2 if args.precision == "fp64" else sig)])
-        detail["compare"] = dict(other_precision=other, other_load_step_s=T2 / K,
-                                 speedup_mixed_over_fp64=(T / T2 if other == "mixed" else T2 / T),
-                                 max_rel_err_sigma_zz=float(np.max(np.abs(mix - ref) / np.abs(ref))),
-                                 other_krylov=[d["iterations"] for d in prob.solver_info])
-        log(f"[bench] {other}: {T2 / K:.3f}s/step, rel err {detail['compare']['max_rel_err_sigma_zz']:.2e}")
+        T2 = (time.perf_counter() - t1) / args.mixed_steps
+        a, b = szz[0], float(sig2[0][2, 2])
+        ref, mix = (a, b) if args.precision == "fp64" else (b, a)
+        detail["compare"] = dict(other_precision=other, other_load_step_s=T2, load_steps=args.mixed_steps,
+                                 speedup_mixed_over_fp64=((T / K) / T2 if other == "mixed" else T2 / (T / K)),
+                                 rel_err_sigma_zz=abs(mix - ref) / abs(ref),
+                                 other_krylov_per_solve=[d["iterations"] for d in prob.solver_info][: 40])
+        log(f"[bench] {other}: {T2:.3f}s/step, speed-up {detail['compare']['speedup_mixed_over_fp64']:.3f}, "
+            f"rel err {detail['compare']['rel_err_sigma_zz']:.2e}")
 
     if args.copy_calibration:
         n = 1 << 26
         src = torch.ones(n, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
-        N.check(L.cpfem_profile(h, 1, 1))
+        from cpfem_amd.problem import stream_handle
+        N.check(prob.ctx._lib.cpfem_profile(prob.ctx.h, 1, 1))
         for _ in range(5):
-            N.check(L.cpfem_copy(h, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream), ctypes.c_void_p(src.data_ptr()),
-                                 ctypes.c_void_p(dst.data_ptr()), n))
-        N.check(L.cpfem_profile_read(h, 8, ms, la, ti))
-        N.check(L.cpfem_profile(h, 0, 1))
-        cms = ms[7] / ti[7]
-        detail["copy_calibration"] = dict(bytes=16 * n, avg_ms=cms, GBps=16 * n / (cms * 1e-3) / 1e9)
+            N.check(prob.ctx._lib.cpfem_copy(prob.ctx.h, stream_handle(), ctypes.c_void_p(src.data_ptr()),
+                                             ctypes.c_void_p(dst.data_ptr()), n))
+        cp, _ = prof.read()
+        detail["copy_calibration"] = dict(bytes=16 * n, avg_ms=cp["copy"]["avg_ms"],
+                                          GBps=16 * n / (cp["copy"]["avg_ms"] * 1e-3) / 1e9)
+        del src, dst
 
-    if args.adjoint:
-        log("[bench] adjoint: calibration-workload history gradient ...")
-        detail["adjoint"] = adjoint_bench(steps=args.adjoint_steps)
-        log(f"[bench] adjoint: {json.dumps(detail['adjoint'])}")
-        log(f"[bench] adjoint: forward {detail['adjoint']['forward_s']:.2f}s, backward {detail['adjoint']['backward_s']:.2f}s")
+    if not args.no_adjoint:
+        if args.config == 5 and S == 0:
+            log("[bench] load-step adjoint (900 Euler angles) ...")
+            a64 = step_adjoint(prob, w, S, szz[0], "fp64")
+            amx = step_adjoint(prob, w, S, szz[0], "mixed")
+            g64, gmx = a64.pop("grad"), amx.pop("grad")
+            a64["mixed"] = dict(grad_s=amx["grad_s"], forward_s=amx["forward_s"], backward_s=amx["backward_s"],
+                                speedup_over_fp64=a64["grad_s"] / amx["grad_s"],
+                                loss_rel_err=abs(amx["loss"] - a64["loss"]) / abs(a64["loss"]),
+                                grad_max_rel_err=float(np.abs(gmx - g64).max() / np.abs(g64).max()))
+            a64.update(n_angles=int(g64.size), grad_norm=float(np.linalg.norm(g64)), szz_rel_err_vs_timed_step=abs(
+                a64["szz"] - szz[0]) / abs(szz[0]))
+            detail["adjoint_load_step"] = a64
+            log(f"[bench] load-step adjoint: {json.dumps(a64)}")
+        log("[bench] history adjoint (calibration workload) ...")
+        detail["adjoint_history"] = history_adjoint(n=args.history_adjoint_mesh)
+        log(f"[bench] history adjoint: {json.dumps(detail['adjoint_history'])}")
 
     cpu = None
     if not args.no_cpu:
-        log("[bench] CPU baseline (oracle sample) ...")
-        cpu = cpu_baseline(prob, w, snap[0], snap[1], per_step)
-        log(f"[bench] cpu {cpu['value']:.1f} {cpu['unit']}")
+        log("[bench] CPU baseline (C++/OpenMP restatement) ...")
+        cpu = cpu_baseline(prob, w, runner.params, [final_sol[0]], S, per_step)
+        detail["speedup_vs_cpu_load_step"] = cpu["load_step_s"] / (T / K)
+        log(f"[bench] cpu {cpu['value']:.1f} {cpu['unit']} ({cpu['load_step_s']:.1f} s per load step)")
 
     line = {
-        "metric": "CP quad-pt updates/s, load-step time, adjoint-grad time (mixed vs FP64)",
+        "metric": METRIC,
         "value": round(value, 1),
         "unit": "quad-pt load-step updates/s",
         "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": round(T / K * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f64+f32-krylov",
         "data": "synthetic (seeded Voronoi polycrystal, uniform SO(3) orientations)",
-        "config": {"workload": w.name, "n_qp": Q, "load_steps": f"{W + 1}..{W + K} of {len(w.ts) - 1}",
-                   "precision": args.precision, "solver": "row-elimination Newton, Jacobi-BiCGSTAB tol=atol=1e-10"},
+        "config": {"workload": w.name, "n_qp": Q, "load_step": f"{S + 1} of {len(w.ts) - 1}, re-run from its snapshot",
+                   "precision": args.precision, "solver": "row-elimination Newton, Jacobi-BiCGSTAB tol=atol=1e-10",
+                   "solver_options": w.solver_options},
         "roofline": roof,
         "cpu_baseline": cpu,
         "detail": detail,
     }
-    print(json.dumps(line), flush=True)
+    print(json.dumps(line, default=float), flush=True)
 
 
 if __name__ == "__main__":

@@ -167,8 +167,31 @@ int cpfem_band_order(int64_t n_nodes, const int32_t* col, const int32_t* nslot, 
 int cpfem_spmv(cpfem_ctx* ctx, void* stream, const double* mat, const double* x, double* y);
 
 /* |v|_2 of a device vector of n_dofs entries, returned on the host (synchronises);
-   *point_status (optional) receives the OR of point status bits since the last call. */
+   *point_status (optional) receives the OR of point status bits (1: a point hit the local
+   iteration cap, 2: a point's local state became non-finite) set by the point kernels queued
+   on `stream` since the last call, which reads and clears them in stream order.  Returns
+   CPFEM_ENONFINITE if the norm is NaN / Inf or a point's state became non-finite, else
+   CPFEM_ELOCAL if a point hit the iteration cap (*out and *point_status are set in both
+   cases).  The reference has no such exits: its residual check asserts on NaN
+   (crystal_plasticity_OR_design/solver.py:416-417) and its local Newton has no cap
+   (models_copper.py:247), so a point that does not converge never returns there. */
 int cpfem_norm(cpfem_ctx* ctx, void* stream, const double* v, double* out, int32_t* point_status);
+
+/* Launch configuration of the Jacobi-BiCGSTAB on this context (no counterpart in the
+   reference: diagnostics of the device solver): out[0] = SpMV form (2: 128-row tiles,
+   1: 64-row tiles, 0: 64-row tiles with every slot's loads in flight), out[1] = SpMV blocks,
+   out[2] = vector-kernel blocks. */
+int cpfem_krylov_config(const cpfem_ctx* ctx, int32_t* out);
+/* Forces the SpMV form of this context's BiCGSTAB (-1: picked by system size, the default).
+   For tests that run two forms on one system; form 0 is refused (CPFEM_EBADARG) on systems
+   with more than three 64-row tiles per CU.  No counterpart in the reference. */
+int cpfem_set_spmv_form(cpfem_ctx* ctx, int32_t form);
+
+/* Work counters of the assembly's point kernel, summed over its launches since the last
+   reset: out[0] = local Newton iterations, out[1] = residual evaluations (synchronises;
+   reset != 0 clears them in stream order).  Diagnostics with no counterpart in the
+   reference: the measured inputs of the point kernel's algorithmic flop count. */
+int cpfem_point_counts(cpfem_ctx* ctx, void* stream, int32_t reset, int64_t* out);
 
 /* y = alpha x + beta y on n_dofs entries (device). */
 int cpfem_axpby(cpfem_ctx* ctx, void* stream, double alpha, const double* x, double beta, double* y);
@@ -208,7 +231,8 @@ int cpfem_transpose(cpfem_ctx* ctx, void* stream, const double* mat, double* mat
 /* Kernel-class timing with HIP events on the launch stream (every `sample`-th launch of a
    class is timed, all are counted); enable resets the counters.  Classes (index):
    0 point update + tangent, 1 element stiffness/residual, 2 matrix gather, 3 residual
-   gather, 4 Krylov SpMV kernels, 5 Krylov vector kernels, 6 point stress/history, 7 copy. */
+   gather, 4 Krylov SpMV kernels, 5 Krylov vector kernels, 6 point stress/history, 7 copy,
+   8 residual replacement of the mixed form. */
 int cpfem_profile(cpfem_ctx* ctx, int32_t enable, int32_t sample);
 int cpfem_profile_read(cpfem_ctx* ctx, int32_t n, double* ms, int64_t* launches, int64_t* timed);
 

@@ -90,10 +90,13 @@ _SIGS = {
     "cpfem_profile": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32]),
     "cpfem_profile_read": (ctypes.c_int, [vp, ctypes.c_int32, vp, vp, vp]),
     "cpfem_copy": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_int64]),
+    "cpfem_point_counts": (ctypes.c_int, [vp, vp, ctypes.c_int32, vp]),
+    "cpfem_krylov_config": (ctypes.c_int, [vp, vp]),
+    "cpfem_set_spmv_form": (ctypes.c_int, [vp, ctypes.c_int32]),
 }
 
 PROF_CLASSES = ("element_qp", "element_K", "gather_matrix", "gather_residual", "krylov_spmv", "krylov_vector",
-                "point_local", "copy")
+                "point_local", "copy", "krylov_replace")
 
 EXPORTED = tuple(_SIGS)
 _lib = None

@@ -41,8 +41,11 @@ class Workload:
         front = lambda p: np.isclose(p[1], 0.0, atol=1e-5)
         bottom = lambda p: np.isclose(p[2], 0.0, atol=1e-5)
         top = lambda p: np.isclose(p[2], L, atol=1e-5)
-        zero = lambda p: 0.0
-        return [[left, front, bottom, top], [0, 1, 2, 2], [zero, zero, zero, (lambda d: (lambda p: d))(disp)]]
+        # constant values written componentwise (one value per point for a coordinate-row
+        # array as for a single point), so that the 64^3 boundary lists need no per-point calls
+        zero = lambda p: np.zeros_like(p[0], dtype=np.float64)
+        const = lambda d: (lambda p: np.full_like(p[0], d, dtype=np.float64))
+        return [[left, front, bottom, top], [0, 1, 2, 2], [zero, zero, zero, const(disp)]]
 
 
 def _mesh(n, L=1.0):
@@ -138,6 +141,63 @@ def config5(seed=0, n=64):
 CONFIGS = {1: config1, 2: config2, 3: config3, 4: config4, 5: config5}
 
 
+# ----------------------------------------------------------------------------- history inputs
+# Inputs of cpfem_amd.simulate (the differentiable whole-history form) for a workload:
+# per-point initial slip resistance g0 (cells, 8, N), the eight per-point material
+# parameters mat8 (cells, 8, 8: xm, gss_a, h, t_sat, r, C11, C12, C44) and rotations.
+
+def _per_point_materials(w):
+    nc = len(w.mesh.cells)
+    if w.phase_inds is None:
+        return [w.material] * 1, np.zeros(nc, dtype=np.int64)
+    return list(w.phase_materials), np.asarray(w.phase_inds, dtype=np.int64)
+
+
+def history_inputs(w, prob=None, device=None):
+    """(g0, mat8) float64 tensors of a workload, per phase where it has phases
+    (models_DPsteel_inhomo.py:55-186), on `device` (default: the current HIP device)."""
+    import torch
+    nc = len(w.mesh.cells)
+    Ns = len(M.load_slip_table(w.material.slip))
+    mats, ph = _per_point_materials(w)
+    table = np.array([[m.xm, m.gss_a, m.h, m.t_sat, m.r, m.C11, m.C12, m.C44] for m in mats], dtype=np.float64)
+    g0s = np.array([m.g0 for m in mats], dtype=np.float64)
+    dev = device if device is not None else torch.device("cuda", torch.cuda.current_device())
+    mat8 = torch.as_tensor(table[ph], device=dev)[:, None, :].expand(nc, 8, 8).contiguous()
+    g0 = torch.as_tensor(g0s[ph], device=dev)[:, None, None].expand(nc, 8, Ns).contiguous()
+    return g0, mat8
+
+
+def history_rotations(w):
+    """(cells, 8, 3, 3) device tensor of the workload's crystal rotations (JAX-style clamped
+    orientation indices, as custom_init)."""
+    import torch
+    from .rotations import get_rot_mat
+    nc = len(w.mesh.cells)
+    inds = np.clip(np.asarray(w.cell_ori_inds, dtype=np.int64), 0, len(w.quat) - 1)
+    R = torch.as_tensor(get_rot_mat(w.quat)[inds], device=torch.device("cuda", torch.cuda.current_device()))
+    return R[:, None].expand(nc, 8, 3, 3)
+
+
+def calibration_inputs(w, prob, theta):
+    """(g0, mat8) of the calibration workload as functions of the five scale factors theta of
+    (g0, h0, g_sat, a, m) (differentiable in theta, on theta's device)."""
+    import torch
+    m = w.material
+    nc = len(w.mesh.cells)
+    Ns = len(M.load_slip_table(m.slip))
+    base = torch.tensor([m.xm, m.gss_a, m.h, m.t_sat, m.r, m.C11, m.C12, m.C44], dtype=torch.float64,
+                        device=theta.device)
+    one = torch.ones((), dtype=torch.float64, device=theta.device)
+    sc = torch.stack([theta[4], theta[3], theta[1], theta[2], one, one, one, one])
+    return (m.g0 * theta[0]).expand(nc, 8, Ns), (base * sc).expand(nc, 8, 8)
+
+
+def calibration_query(seed=0):
+    """Query scale factors of the calibration workload: hidden x (1 + U(-0.2, 0.2)), seeded."""
+    return 1.0 + np.random.default_rng(seed).uniform(-0.2, 0.2, size=5)
+
+
 def build_problem(w):
     """CrystalPlasticity for a workload (BCs at the first displacement)."""
     from .models import CrystalPlasticity
@@ -160,7 +220,7 @@ def run_steps(prob, w, params, sol, i0, i1, solver_options=None, on_step=None):
     info = prob._bc_info
     for i in range(i0, i1):
         prob.dt = w.ts[i + 1] - w.ts[i]
-        info[-1][-1] = (lambda d: (lambda p: d))(w.disps[i + 1])
+        info[-1][-1] = w.dirichlet_bc_info(w.disps[i + 1])[-1][-1]
         prob.fes[0].update_Dirichlet_boundary_conditions(info)
         prob.set_params(params)
         o = {"initial_guess": sol}

@@ -110,13 +110,15 @@ class BlockEllMatrix:
 
 
 def _eval_location(fn, points):
-    """Vectorised evaluation of a reference location / value function over mesh points."""
+    """A reference location / value function at every mesh point, with the per-point
+    semantics of jax-fem's ``vmap(fn)(points)``.  The function is first called once on the
+    coordinate rows (points.T): componentwise functions return one value per point there.
+    Anything else (an exception, or a 0-d result: a constant, or a function that reduces over
+    its point such as ``np.all(np.isclose(p, c))``) is evaluated point by point."""
     try:
         v = np.asarray(fn(points.T))
         if v.shape == (len(points),):
             return v
-        if v.ndim == 0:
-            return np.full(len(points), v.item())
     except Exception:
         pass
     return np.array([np.asarray(fn(p)).item() for p in points])

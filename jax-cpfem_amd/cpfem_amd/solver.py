@@ -28,12 +28,23 @@ from .problem import BlockEllMatrix, device, ptr, stream_handle
 logger = logging.getLogger("cpfem_amd")
 
 
-def norm(problem, v, status_out=None):
+def norm(problem, v, status_out=None, allow_nonfinite=False):
+    """|v|_2 of a residual, with the point-status check of the kernels queued before it.
+
+    A point that hit the local iteration cap raises (CpfemError, code ELOCAL): the
+    reference's local Newton has no cap and would never return there.  A non-finite residual
+    or point state raises (code ENONFINITE) unless ``allow_nonfinite``, in which case NaN is
+    returned, as np.linalg.norm does in the reference's Newton loop and line search
+    (solver.py:247-249,408-416)."""
     out = ctypes.c_double()
     st = ctypes.c_int32()
-    N.check(problem.ctx._lib.cpfem_norm(problem.ctx.h, stream_handle(), ptr(v), ctypes.byref(out), ctypes.byref(st)))
+    rc = problem.ctx._lib.cpfem_norm(problem.ctx.h, stream_handle(), ptr(v), ctypes.byref(out), ctypes.byref(st))
     if status_out is not None:
         status_out.append(int(st.value))
+    if rc == N.ENONFINITE and allow_nonfinite:
+        logger.debug("non-finite residual: %s", problem.ctx._lib.cpfem_last_error().decode(errors="replace"))
+        return math.nan
+    N.check(rc)
     return out.value
 
 
@@ -100,7 +111,7 @@ def line_search(problem, dofs, inc):
     def res_norm(alpha):
         d = dofs + alpha * inc
         res, _ = problem.assemble(d, want_matrix=False, apply_bc=True)
-        return norm(problem, res)
+        return norm(problem, res, allow_nonfinite=True)
 
     alpha = 1.0
     rn = res_norm(alpha)
@@ -142,7 +153,7 @@ def solver(problem, solver_options={}):
     tol = solver_options.get("tol", 1e-6)
     status = []
     res, mat = problem.assemble(dofs)
-    res_val = norm(problem, res, status)
+    res_val = norm(problem, res, status, allow_nonfinite=True)
     res_val_initial = res_val
     rel_res_val = res_val / res_val_initial if res_val_initial != 0 else math.nan
     logger.debug(f"Before, l_2 res = {res_val}, relative l_2 res = {rel_res_val}")
@@ -153,20 +164,23 @@ def solver(problem, solver_options={}):
             raise RuntimeError(f"Newton did not converge in {max_newton} iterations (l_2 res = {res_val})")
         dofs = linear_incremental_solver(problem, res, mat, dofs, solver_options)
         res, mat = problem.assemble(dofs)
-        res_val = norm(problem, res, status)
+        res_val = norm(problem, res, status, allow_nonfinite=True)
         rel_res_val = res_val / res_val_initial
         newton += 1
         logger.debug(f"l_2 res = {res_val}, relative l_2 res = {rel_res_val}")
     if not math.isfinite(res_val):
-        flags = int(max(status) if status else 0)
+        flags = 0
+        for s in status:
+            flags |= s
         logger.error("non-finite residual after %d Newton iterations; point flags: %s", newton,
                      ", ".join(n for b, n in ((N.PT_CAPPED, "local Newton hit its iteration cap"),
                                               (N.PT_NONFINITE, "non-finite local state")) if flags & b) or "none")
+    # solver.py:416-417: a NaN residual ends the loop above and fails here
     assert math.isfinite(res_val), "res_val contains NaN, stop the program!"
     assert bool(torch.isfinite(dofs).all()), "dofs contains NaN, stop the program!"
-    problem.last_solve = dict(newton=newton, point_status=int(max(status) if status else 0),
-                              seconds=time.time() - start)
-    if problem.last_solve["point_status"] & N.PT_CAPPED:
-        logger.warning("some quadrature points hit the local Newton iteration cap")
+    flags = 0
+    for s in status:
+        flags |= s
+    problem.last_solve = dict(newton=newton, point_status=flags, seconds=time.time() - start)
     logger.info(f"Solve took {time.time() - start} [s]")
     return problem.unflatten_fn_sol_list(dofs)

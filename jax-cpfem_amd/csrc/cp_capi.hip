@@ -117,6 +117,7 @@ struct cpfem_ctx {
   // device: element work buffers
   double *d_wP = nullptr, *d_wA = nullptr, *d_KE = nullptr, *d_RE = nullptr, *d_sig = nullptr;
   int32_t* d_status = nullptr;
+  unsigned long long* d_counts = nullptr;  // [0] local Newton iterations, [1] residual evaluations
   // reductions
   int nb = 0;
   double* d_partial = nullptr;
@@ -132,12 +133,16 @@ struct cpfem_ctx {
   KrylovScalars* h_ksc = nullptr;
   double* h_dbl = nullptr;
   int32_t* h_int = nullptr;
+  unsigned long long* h_cnt = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
   // banded direct solver (allocated on first use)
   BandLU band;
   int32_t* d_ip = nullptr;
   int32_t* d_bflag = nullptr;
   Prof prof;
+  int spmv_form = -1;  // BiCGSTAB SpMV form: -1 = picked by size (krylov_spmv_form), else forced
+
+  int krylov_form() const { return spmv_form >= 0 ? spmv_form : krylov_spmv_form(n_dofs); }
 
   Geom geom() const {
     Geom G;
@@ -387,7 +392,7 @@ static int ensure_krylov(cpfem_ctx* c, bool mixed) {
   int rc;
   const int64_t n = c->n_dofs;
   if (!c->d_ksc) {
-    if ((rc = dalloc(&c->d_ksc, 1)) || (rc = dalloc(&c->d_kpartial, (size_t)8 * krylov_stride(n)))) return rc;
+    if ((rc = dalloc(&c->d_ksc, 1)) || (rc = dalloc(&c->d_kpartial, (size_t)8 * krylov_stride_max(n)))) return rc;
     for (int i = 0; i < 12; ++i)
       if ((rc = dalloc(&c->kv64[i], i >= 10 ? 2 * n : n))) return rc;
     if ((rc = dalloc(&c->d_r64, n)) || (rc = dalloc(&c->d_tmp64, n))) return rc;
@@ -441,9 +446,9 @@ static int run_bicgstab(cpfem_ctx* c, hipStream_t st, const TM* vals, const doub
   KrylovWork w;
   w.n = c->n_dofs;
   w.nb = krylov_blocks(c->n_dofs);
-  w.nbs = krylov_spmv_blocks(c->n_dofs);
-  w.sf = krylov_spmv_form(c->n_dofs);
-  w.ps = krylov_stride(c->n_dofs);
+  w.sf = c->krylov_form();
+  w.nbs = krylov_spmv_blocks(c->n_dofs, w.sf);
+  w.ps = std::max(w.nb, w.nbs);
   w.partial = c->d_kpartial;
   w.sc = c->d_ksc;
   KVec v = kvec(c->kv64, x);
@@ -458,7 +463,7 @@ static int run_bicgstab(cpfem_ctx* c, hipStream_t st, const TM* vals, const doub
   while (!done) {
     for (int i = 0; i < CH; ++i) krylov_iteration<TM>(M, vals, v, w, i & 1, st, &c->prof);
     if (vals64) {
-      const int sr = prof_begin(&c->prof, PC_SPMV, st);
+      const int sr = prof_begin(&c->prof, PC_KREPLACE, st);
       krylov_replace(M, vals64, b, v, w, batch, st);
       prof_end(&c->prof, sr, st);
     }
@@ -525,7 +530,7 @@ int cpfem_create(const CpfemMesh* mesh, const CpfemSlip* slip, cpfem_ctx** out) 
   c->nb = std::max(c->nb, 1);
   if ((rc = dalloc(&c->d_isbc, c->n_dofs)) || (rc = dalloc(&c->d_nobc, c->n_dofs)) || (rc = dalloc(&c->d_ubar, c->n_dofs)) ||
       (rc = dalloc(&c->d_wP, (size_t)9 * c->n_qp)) || (rc = dalloc(&c->d_RE, (size_t)24 * c->n_cells)) ||
-      (rc = dalloc(&c->d_sig, (size_t)9 * c->n_qp)) || (rc = dalloc(&c->d_status, 1)) ||
+      (rc = dalloc(&c->d_sig, (size_t)9 * c->n_qp)) || (rc = dalloc(&c->d_status, 1)) || (rc = dalloc(&c->d_counts, 2)) ||
       (rc = dalloc(&c->d_partial, (size_t)4 * std::max(c->nb, nbq))) || (rc = dalloc(&c->d_scalar, 4)) ||
       (rc = dalloc(&c->d_counter, 4))) {
     const std::string e = g_err;
@@ -535,9 +540,11 @@ int cpfem_create(const CpfemMesh* mesh, const CpfemSlip* slip, cpfem_ctx** out) 
   }
   if (hipMemset(c->d_isbc, 0, c->n_dofs) != hipSuccess || hipMemset(c->d_nobc, 0, c->n_dofs) != hipSuccess || hipMemset(c->d_ubar, 0, 8 * c->n_dofs) != hipSuccess ||
       hipMemset(c->d_status, 0, 4) != hipSuccess || hipMemset(c->d_counter, 0, 16) != hipSuccess ||
+      hipMemset(c->d_counts, 0, 16) != hipSuccess ||
       hipHostMalloc((void**)&c->h_ksc, sizeof(KrylovScalars)) != hipSuccess ||
       hipHostMalloc((void**)&c->h_dbl, 8 * sizeof(double)) != hipSuccess ||
       hipHostMalloc((void**)&c->h_int, 8 * sizeof(int32_t)) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_cnt, 2 * sizeof(unsigned long long)) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev[0], hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev[1], hipEventDisableTiming) != hipSuccess) {
     cpfem_destroy(c);
@@ -555,7 +562,7 @@ void cpfem_destroy(cpfem_ctx* c) {
   dfree(c->d_col); dfree(c->d_nslot); dfree(c->d_dslot); dfree(c->d_tslot);
   dfree(c->d_cptr); dfree(c->d_clist); dfree(c->d_nptr); dfree(c->d_nlist);
   dfree(c->d_isbc); dfree(c->d_nobc); dfree(c->d_ubar);
-  dfree(c->d_wP); dfree(c->d_wA); dfree(c->d_KE); dfree(c->d_RE); dfree(c->d_sig); dfree(c->d_status);
+  dfree(c->d_wP); dfree(c->d_wA); dfree(c->d_KE); dfree(c->d_RE); dfree(c->d_sig); dfree(c->d_status); dfree(c->d_counts);
   dfree(c->d_partial); dfree(c->d_scalar); dfree(c->d_counter);
   dfree(c->d_ksc);
   dfree(c->d_kpartial);
@@ -564,6 +571,7 @@ void cpfem_destroy(cpfem_ctx* c) {
   if (c->h_ksc) (void)hipHostFree(c->h_ksc);
   if (c->h_dbl) (void)hipHostFree(c->h_dbl);
   if (c->h_int) (void)hipHostFree(c->h_int);
+  if (c->h_cnt) (void)hipHostFree(c->h_cnt);
   for (int i = 0; i < 2; ++i)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
   prof_flush(&c->prof);
@@ -661,9 +669,9 @@ int cpfem_assemble(cpfem_ctx* c, void* stream, const double* dofs, const CpfemSt
   Prof* pf = &c->prof;
   int sl = prof_begin(pf, PC_ELEMENT_QP, st);
   if (c->n_slip == 12)
-    launch_element_qp<12>(G, dofs, sv, mv, c->tab12, cfg, c->d_wP, wA, iters, c->d_status, st);
+    launch_element_qp<12>(G, dofs, sv, mv, c->tab12, cfg, c->d_wP, wA, iters, c->d_status, c->d_counts, st);
   else
-    launch_element_qp<24>(G, dofs, sv, mv, c->tab24, cfg, c->d_wP, wA, iters, c->d_status, st);
+    launch_element_qp<24>(G, dofs, sv, mv, c->tab24, cfg, c->d_wP, wA, iters, c->d_status, c->d_counts, st);
   prof_end(pf, sl, st);
   sl = prof_begin(pf, PC_ELEMENT_K, st);
   launch_element_K(G, c->d_wP, wA, c->d_KE, c->d_RE, st);
@@ -695,6 +703,8 @@ int cpfem_cell_stress(cpfem_ctx* c, void* stream, const double* dofs, const Cpfe
 int cpfem_update_state(cpfem_ctx* c, void* stream, const double* dofs, const CpfemState* state, const CpfemParams* params,
                        const CpfemStateOut* out) {
   if (!c || !dofs || !out) return fail(CPFEM_EBADARG, "null argument");
+  int rc;
+  if ((rc = check_state(c, state, params))) return rc;
   if (out->Fp_inv == state->Fp_inv || out->g == state->g || out->slip == state->slip)
     return fail(CPFEM_EBADARG, "state_out must not alias state");
   return cpfem_local(c, stream, c->n_qp, nullptr, dofs, state, params, nullptr, nullptr, nullptr, out, nullptr);
@@ -780,15 +790,50 @@ int cpfem_spmv(cpfem_ctx* c, void* stream, const double* mat, const double* x, d
 int cpfem_norm(cpfem_ctx* c, void* stream, const double* v, double* out, int32_t* point_status) {
   if (!c || !v || !out) return fail(CPFEM_EBADARG, "null argument");
   hipStream_t st = (hipStream_t)stream;
+  // status word: read and cleared in stream order, behind every point kernel queued so far
+  HIPCHK(hipMemcpyAsync(&c->h_int[4], c->d_status, 4, hipMemcpyDeviceToHost, st));
+  launch_zero_words(reinterpret_cast<uint32_t*>(c->d_status), 1, st);
   double s;
   int rc;
-  if ((rc = dot_host(c, st, v, v, &s))) return rc;
+  if ((rc = dot_host(c, st, v, v, &s))) return rc;  // synchronises st
   *out = std::sqrt(s);
-  if (point_status) {
-    HIPCHK(hipMemcpy(c->h_int, c->d_status, 4, hipMemcpyDeviceToHost));
-    *point_status = c->h_int[0];
-    HIPCHK(hipMemset(c->d_status, 0, 4));
-  }
+  const int32_t bits = c->h_int[4];
+  if (point_status) *point_status = bits;
+  const std::string flags = std::string((bits & PT_CAPPED) ? " [local Newton iteration cap reached]" : "") +
+                            ((bits & PT_NONFINITE) ? " [non-finite local state]" : "");
+  if (!std::isfinite(*out)) return fail(CPFEM_ENONFINITE, "non-finite residual norm" + flags);
+  if (bits & PT_CAPPED)
+    return fail(CPFEM_ELOCAL, "some quadrature points hit the local Newton iteration cap" + flags);
+  if (bits & PT_NONFINITE) return fail(CPFEM_ENONFINITE, "non-finite local state at some quadrature points" + flags);
+  return CPFEM_OK;
+}
+
+int cpfem_krylov_config(const cpfem_ctx* c, int32_t* out) {
+  if (!c || !out) return fail(CPFEM_EBADARG, "null argument");
+  out[0] = c->krylov_form();
+  out[1] = krylov_spmv_blocks(c->n_dofs, out[0]);
+  out[2] = krylov_blocks(c->n_dofs);
+  return CPFEM_OK;
+}
+
+int cpfem_set_spmv_form(cpfem_ctx* c, int32_t form) {
+  if (!c) return fail(CPFEM_EBADARG, "null context");
+  if (form < -1 || form > 2) return fail(CPFEM_EBADARG, "SpMV form must be -1 (by size), 0, 1 or 2");
+  if (form >= 0 && !krylov_form_allowed(c->n_dofs, form))
+    return fail(CPFEM_EBADARG, "the all-slots SpMV form needs at most three 64-row tiles per CU");
+  c->spmv_form = form;
+  return CPFEM_OK;
+}
+
+int cpfem_point_counts(cpfem_ctx* c, void* stream, int32_t reset, int64_t* out) {
+  if (!c || !out) return fail(CPFEM_EBADARG, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(hipMemcpyAsync(c->h_cnt, c->d_counts, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+  if (reset) launch_zero_words(reinterpret_cast<uint32_t*>(c->d_counts), 4, st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
+  out[0] = (int64_t)c->h_cnt[0];
+  out[1] = (int64_t)c->h_cnt[1];
   return CPFEM_OK;
 }
 

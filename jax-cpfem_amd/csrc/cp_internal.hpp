@@ -59,7 +59,8 @@ struct BellView {
 
 template <int N>
 void launch_element_qp(const Geom& G, const double* u, const StateView& sv, const MatView& mv, const SlipTable<N>& tab,
-                       const SolveCfg& cfg, double* wP, double* wA, int32_t* iters, int32_t* status, hipStream_t st);
+                       const SolveCfg& cfg, double* wP, double* wA, int32_t* iters, int32_t* status,
+                       unsigned long long* counts, hipStream_t st);
 template <int N>
 void launch_local(int64_t n_qp, const double* Hd, const Geom& G, const double* u, const StateView& sv, const MatView& mv,
                   const SlipTable<N>& tab, const SolveCfg& cfg, int self_block, double* P, double* A, double* sig,
@@ -71,6 +72,8 @@ void launch_gather_residual(int64_t n_nodes, const double* RE, const int32_t* np
                             const double* ubar, const double* u, double* res, hipStream_t st);
 void launch_cell_average(const Geom& G, const double* sig, double* out, hipStream_t st);
 void launch_bc_x0(int64_t n, const uint8_t* is_bc, const double* ubar, const double* u, double* x0, hipStream_t st);
+// Zero n 32-bit words with a kernel (stream-ordered and safe inside captured graphs).
+void launch_zero_words(uint32_t* p, int64_t n, hipStream_t st);
 
 // ---------------------------------------------------------------- adjoint
 struct VjpArgs {
@@ -106,6 +109,7 @@ enum ProfClass {
   PC_KVEC,            // Krylov vector kernels (k_kp, k_ks, k_kxr)
   PC_LOCAL,           // point kernel for stress / history (k_local)
   PC_COPY,            // plain copy (bandwidth calibration)
+  PC_KREPLACE,        // residual replacement of the mixed form (k_kreplace)
   PC_COUNT
 };
 
@@ -191,8 +195,10 @@ template <typename TM>
 void krylov_iteration(const BellView& M, const TM* vals, KVec v, KrylovWork w, int par, hipStream_t st, Prof* prof);
 int krylov_blocks(int64_t n);
 int krylov_spmv_blocks(int64_t n);
+int krylov_spmv_blocks(int64_t n, int form);
 int krylov_spmv_form(int64_t n);
-inline int krylov_stride(int64_t n) { return krylov_blocks(n) > krylov_spmv_blocks(n) ? krylov_blocks(n) : krylov_spmv_blocks(n); }
+bool krylov_form_allowed(int64_t n, int form);
+int krylov_stride_max(int64_t n);
 void krylov_replace(const BellView& M, const double* vals64, const double* b, KVec v, KrylovWork w, int offer, hipStream_t st);
 void launch_spmv(const BellView& M, const double* vals, const double* x, double* y, hipStream_t st);
 void launch_inv_diag(const BellView& M, const double* vals, const int32_t* dslot, double* invd, int precond, hipStream_t st);

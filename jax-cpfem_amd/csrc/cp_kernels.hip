@@ -57,44 +57,63 @@ __device__ __forceinline__ void load_point(const StateView& sv, int64_t q, doubl
 
 constexpr int QBS = 128;  // threads per block of the point kernels (LDS: 2 * N * QBS doubles)
 
+// Sum of a 32-bit count over the 64 lanes of a wave (every lane takes part).
+__device__ __forceinline__ uint32_t wave_count(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 // ---------------------------------------------------------------------------------
 // Quadrature-point kernel of the assembly: H from dofs, local Newton, JxW-weighted P (SoA
 // [9][Q]) and, if wA, the JxW-weighted consistent tangent stored per point ([Q][81]: the
-// element kernel's 8 threads of a cell then read whole cache lines together).
+// element kernel's 8 threads of a cell then read whole cache lines together).  counts
+// (optional): [0] += local Newton iterations, [1] += residual evaluations over the launch
+// (one atomic per wave), the measured inputs of the algorithmic flop count.
 template <int N>
 __global__ void __launch_bounds__(QBS) k_element_qp(Geom G, const double* __restrict__ u, StateView sv, MatView mv,
                                                    SlipTable<N> tab, SolveCfg cfg, double* __restrict__ wP,
                                                    double* __restrict__ wA, int32_t* __restrict__ iters,
-                                                   int32_t* __restrict__ status) {
+                                                   int32_t* __restrict__ status, unsigned long long* __restrict__ counts) {
   __shared__ double sh[2 * N * QBS];
   const int64_t q = (int64_t)blockIdx.x * QBS + threadIdx.x;
-  if (q >= G.n_qp) return;
-  double H[9], F[9], Fpi[9], R[9];
-  ugrad_from_dofs(G, u, q, H);
+  uint32_t n_it = 0, n_ev = 0;
+  if (q < G.n_qp) {
+    double H[9], F[9], Fpi[9], R[9];
+    ugrad_from_dofs(G, u, q, H);
 #pragma unroll
-  for (int i = 0; i < 9; ++i) F[i] = H[i] + ((i % 4) == 0 ? 1.0 : 0.0);
-  load_point(sv, q, Fpi, R);
-  const PointMat m = load_mat(mv, q);
-  Point<N> pt;
-  pt.setup(F, Fpi, R, sv.g + q, sv.ld, m, sh + threadIdx.x, sh + N * QBS + threadIdx.x, QBS);
-  double s[6];
-  int32_t st = 0;
-  const int it = pt.solve(tab, cfg.tol, cfg.max_sub_step, cfg.max_iter, s, &st);
-  const double w = jxw_at(G, q);
-  double P[9];
-  pk1<N>(pt, F, Fpi, s, P);
+    for (int i = 0; i < 9; ++i) F[i] = H[i] + ((i % 4) == 0 ? 1.0 : 0.0);
+    load_point(sv, q, Fpi, R);
+    const PointMat m = load_mat(mv, q);
+    Point<N> pt;
+    pt.setup(F, Fpi, R, sv.g + q, sv.ld, m, sh + threadIdx.x, sh + N * QBS + threadIdx.x, QBS);
+    double s[6];
+    int32_t st = 0;
+    const int it = pt.solve(tab, cfg.tol, cfg.max_sub_step, cfg.max_iter, s, &st);
+    const double w = jxw_at(G, q);
+    double P[9];
+    pk1<N>(pt, F, Fpi, s, P);
 #pragma unroll
-  for (int i = 0; i < 9; ++i) wP[i * G.n_qp + q] = w * P[i];
-  if (wA) {
-    const int64_t Q = G.n_qp;
-    (void)Q;
-    tangent<N>(pt, tab, F, Fpi, s, [&](int m9, const double* dP) {
+    for (int i = 0; i < 9; ++i) wP[i * G.n_qp + q] = w * P[i];
+    if (wA) {
+      tangent<N>(pt, tab, F, Fpi, s, [&](int m9, const double* dP) {
 #pragma unroll
-      for (int i = 0; i < 9; ++i) wA[q * 81 + 9 * i + m9] = w * dP[i];
-    });
+        for (int i = 0; i < 9; ++i) wA[q * 81 + 9 * i + m9] = w * dP[i];
+      });
+    }
+    if (iters) iters[q] = it;
+    if (st) atomicOr(status, st);
+    n_it = (uint32_t)it;
+    n_ev = (uint32_t)pt.nev;
   }
-  if (iters) iters[q] = it;
-  if (st) atomicOr(status, st);
+  if (counts) {
+    n_it = wave_count(n_it);
+    n_ev = wave_count(n_ev);
+    if ((threadIdx.x & 63) == 0) {
+      atomicAdd(&counts[0], (unsigned long long)n_it);
+      atomicAdd(&counts[1], (unsigned long long)n_ev);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------------
@@ -323,9 +342,11 @@ __global__ void k_bc_x0(int64_t n, const uint8_t* __restrict__ is_bc, const doub
 // Launch helpers (host)
 template <int N>
 void launch_element_qp(const Geom& G, const double* u, const StateView& sv, const MatView& mv, const SlipTable<N>& tab,
-                       const SolveCfg& cfg, double* wP, double* wA, int32_t* iters, int32_t* status, hipStream_t st) {
+                       const SolveCfg& cfg, double* wP, double* wA, int32_t* iters, int32_t* status,
+                       unsigned long long* counts, hipStream_t st) {
   const int64_t nb = (G.n_qp + QBS - 1) / QBS;
-  hipLaunchKernelGGL(k_element_qp<N>, dim3((unsigned)nb), dim3(QBS), 0, st, G, u, sv, mv, tab, cfg, wP, wA, iters, status);
+  hipLaunchKernelGGL(k_element_qp<N>, dim3((unsigned)nb), dim3(QBS), 0, st, G, u, sv, mv, tab, cfg, wP, wA, iters, status,
+                     counts);
 }
 
 template <int N>
@@ -369,6 +390,15 @@ void launch_bc_x0(int64_t n, const uint8_t* is_bc, const double* ubar, const dou
   hipLaunchKernelGGL(k_bc_x0, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, is_bc, ubar, u, x0);
 }
 
+__global__ void k_zero_words(uint32_t* __restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+void launch_zero_words(uint32_t* p, int64_t n, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_zero_words, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+}
+
 void launch_cell_average(const Geom& G, const double* sig, double* out, hipStream_t st) {
   const int bs = 256;
   const int64_t n = G.n_cells * 9;
@@ -377,7 +407,8 @@ void launch_cell_average(const Geom& G, const double* sig, double* out, hipStrea
 
 #define CPFEM_INST(N)                                                                                                      \
   template void launch_element_qp<N>(const Geom&, const double*, const StateView&, const MatView&, const SlipTable<N>&,    \
-                                     const SolveCfg&, double*, double*, int32_t*, int32_t*, hipStream_t);                  \
+                                     const SolveCfg&, double*, double*, int32_t*, int32_t*, unsigned long long*,           \
+                                     hipStream_t);                                                                         \
   template void launch_local<N>(int64_t, const double*, const Geom&, const double*, const StateView&, const MatView&,      \
                                 const SlipTable<N>&, const SolveCfg&, int, double*, double*, double*, const StateOut&,     \
                                 int32_t*, int32_t*, hipStream_t);

@@ -872,10 +872,23 @@ int krylov_spmv_form(int64_t n) {
 // 64-row tile gets a block of its own; a grid beyond that runs in two rounds: on the 32^3
 // matrix with 1024-thread blocks, 256 blocks took 46.3 us per BiCGSTAB iteration, 384 blocks
 // 61.7 us, 422 blocks 50.4 us).
-int krylov_spmv_blocks(int64_t n) {
-  if (krylov_spmv_form(n) == 2)
+int krylov_spmv_blocks(int64_t n, int form) {
+  if (form == 2)
     return (int)std::max<int64_t>(1, std::min<int64_t>((n + SpG<2>::BLOCK_ROWS - 1) / SpG<2>::BLOCK_ROWS, 2 * device_cus()));
   return (int)std::max<int64_t>(1, std::min<int64_t>((n + SpG<1>::BLOCK_ROWS - 1) / SpG<1>::BLOCK_ROWS, 4 * device_cus()));
+}
+int krylov_spmv_blocks(int64_t n) { return krylov_spmv_blocks(n, krylov_spmv_form(n)); }
+
+// Whether a form may run on n rows: the all-slots form only where every 64-row tile gets a
+// block of its own at three blocks per CU (its lane offsets are 32-bit on that assumption).
+bool krylov_form_allowed(int64_t n, int form) {
+  if (form == 1 || form == 2) return true;
+  return form == 0 && (n + SpG<1>::BLOCK_ROWS - 1) / SpG<1>::BLOCK_ROWS <= 3 * device_cus();
+}
+
+// Partial-sum stride that holds every form's grid (the form can be set per context).
+int krylov_stride_max(int64_t n) {
+  return std::max(krylov_blocks(n), std::max(krylov_spmv_blocks(n, 1), krylov_spmv_blocks(n, 2)));
 }
 
 void krylov_replace(const BellView& M, const double* vals64, const double* b, KVec v, KrylovWork w, int offer,

@@ -217,6 +217,7 @@ struct Point {
   double Fe[9];
   double r[6];
   double nr;
+  int nev;        // residual evaluations since setup (work counter)
 
   CPFEM_HD double& gp(int a) { return gpv[a * st]; }
   CPFEM_HD double& gi(int a) { return giv[a * st]; }
@@ -234,6 +235,7 @@ struct Point {
     for (int a = 0; a < N; ++a) gi(a) = 1.0 / g_old[a * gstride];
     p = m.p;
     c = m.c;
+    nev = 0;
   }
 
   // crystal-frame stress R^T S R of the symmetric Voigt stress s
@@ -246,6 +248,7 @@ struct Point {
 
   // Residual at symmetric S (Voigt 6); fills gp, Lpc, Fe, r, nr.
   CPFEM_HD void eval(const SlipTable<N>& tab, const double* s) {
+    ++nev;
     double Sc[9];
     crystal_stress(s, Sc);
 #pragma unroll

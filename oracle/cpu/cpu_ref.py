@@ -1,0 +1,188 @@
+"""ctypes front end of the C++/OpenMP CPU restatement (oracle/cpu/cpfem_cpu.cpp).
+
+TEST INFRASTRUCTURE ONLY: the benchmark's CPU-baseline leg and the CPU tests use it; the
+product path never does.
+"""
+import ctypes
+import os
+import time
+
+import numpy as np
+
+from . import build as _build
+
+_lib = None
+vp = ctypes.c_void_p
+
+
+def _host_has_v3():
+    try:
+        flags = set()
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("flags"):
+                flags = set(line.split(":", 1)[1].split())
+                break
+        return {"avx2", "fma", "bmi2"} <= flags
+    except OSError:
+        return False
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = _build.LIBS["v3"] if _host_has_v3() else _build.LIBS["base"]
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: run __graft_entry__.build()")
+        L = ctypes.CDLL(path)
+        L.cpu_threads.restype = ctypes.c_int
+        L.cpu_create.restype = vp
+        L.cpu_create.argtypes = [ctypes.c_int64, ctypes.c_int64, vp, vp, ctypes.c_int, vp, vp]
+        L.cpu_destroy.argtypes = [vp]
+        L.cpu_nnz.restype = ctypes.c_int64
+        L.cpu_nnz.argtypes = [vp]
+        L.cpu_csr.argtypes = [vp, vp, vp]
+        state = [vp, vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                 ctypes.c_int]
+        L.cpu_assemble.argtypes = [vp, vp] + state + [vp, vp, vp, vp, vp]
+        L.cpu_point_pass.argtypes = [vp, vp] + state + [vp, vp, vp]
+        L.cpu_vjp_pass.argtypes = [vp, vp] + state + [vp, vp, vp]
+        L.cpu_bicgstab.restype = ctypes.c_int
+        L.cpu_bicgstab.argtypes = [vp, vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def threads():
+    return int(lib().cpu_threads())
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class Model:
+    """Mesh, per-point state and material of one load step on the host."""
+
+    def __init__(self, points, cells, directions, normals, Fpi, g, R, mat8, ao, tol, max_sub, max_iter, self_block,
+                 bc_dofs, bc_vals):
+        L = lib()
+        self.points = np.ascontiguousarray(points, dtype=np.float64)
+        self.cells = np.ascontiguousarray(cells, dtype=np.int32)
+        d = np.ascontiguousarray(directions, dtype=np.float64)
+        n = np.ascontiguousarray(normals, dtype=np.float64)
+        self.h = L.cpu_create(len(self.points), len(self.cells), _p(self.points), _p(self.cells), len(d), _p(d), _p(n))
+        if not self.h:
+            raise ValueError("unsupported slip system count")
+        self.n_dofs = 3 * len(self.points)
+        self.Q = 8 * len(self.cells)
+        self.Fpi = np.ascontiguousarray(Fpi, dtype=np.float64).reshape(9, self.Q)
+        self.g = np.ascontiguousarray(g, dtype=np.float64).reshape(-1, self.Q)
+        self.R = np.ascontiguousarray(R, dtype=np.float64).reshape(9, self.Q)
+        self.mat8 = np.ascontiguousarray(mat8, dtype=np.float64).reshape(8, self.Q)
+        self.ao, self.tol, self.max_sub, self.max_iter, self.self_block = ao, tol, max_sub, max_iter, self_block
+        self.is_bc = np.zeros(self.n_dofs, dtype=np.uint8)
+        self.ubar = np.zeros(self.n_dofs)
+        self.is_bc[np.asarray(bc_dofs, dtype=np.int64)] = 1
+        self.ubar[np.asarray(bc_dofs, dtype=np.int64)] = bc_vals
+        nnz = L.cpu_nnz(self.h)
+        self.indptr = np.empty(self.n_dofs + 1, dtype=np.int64)
+        self.indices = np.empty(nnz, dtype=np.int32)
+        L.cpu_csr(self.h, _p(self.indptr), _p(self.indices))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().cpu_destroy(self.h)
+            self.h = None
+
+    @classmethod
+    def from_problem(cls, prob, params, w=None):
+        """From a cpfem_amd CrystalPlasticity problem and its internal_vars (device tensors
+        copied to the host once)."""
+        import torch
+        fe = prob.fes[0]
+        nc, Ns, Q = fe.num_cells, prob.num_slip_sys, 8 * fe.num_cells
+        host = lambda t, comps: torch.as_tensor(t).reshape((nc, 8) + comps).permute(
+            *range(2, 2 + len(comps)), 0, 1).reshape(-1, Q).cpu().numpy()
+        m = prob.material
+        mat8 = np.empty((8, Q))
+        scal = [m.xm, m.gss_a, m.h, m.t_sat, m.r, m.C11, m.C12, m.C44]
+        for k in range(8):
+            mat8[k] = scal[k]
+        if len(params) >= 9:  # 9-array order gss_a, h, t_sat, xm, r
+            for k, idx in ((1, 4), (2, 5), (3, 6), (0, 7), (4, 8)):
+                mat8[k] = torch.as_tensor(params[idx]).reshape(Q).cpu().numpy()
+        if len(params) >= 10:
+            C = torch.as_tensor(params[9]).reshape(Q, 3, 3, 3, 3)
+            mat8[5] = C[:, 0, 0, 0, 0].cpu().numpy()
+            mat8[6] = C[:, 0, 0, 1, 1].cpu().numpy()
+            mat8[7] = C[:, 1, 2, 1, 2].cpu().numpy()
+        dofs, vals = fe.dirichlet_dofs()
+        return cls(fe.points, fe.cells, prob.slip_directions, prob.slip_normals, host(params[0], (3, 3)),
+                   host(params[1], (Ns,)), host(params[3], (3, 3)), mat8, m.ao, m.tol, m.max_sub_step, m.max_iter,
+                   1 if m.self_block else 0, dofs, vals)
+
+    def _state(self, dt):
+        return [_p(self.Fpi), _p(self.g), _p(self.R), _p(self.mat8), self.ao, float(dt), self.tol, self.max_sub,
+                self.max_iter, self.self_block]
+
+    def assemble(self, u, dt, want_matrix=True):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        res = np.empty(self.n_dofs)
+        vals = np.empty(len(self.indices)) if want_matrix else None
+        counts = np.zeros(2, dtype=np.int64)
+        lib().cpu_assemble(self.h, _p(u), *self._state(dt), _p(self.is_bc), _p(self.ubar), _p(res),
+                           _p(vals) if want_matrix else None, _p(counts))
+        return res, vals, counts
+
+    def point_pass(self, u, dt):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        sig = np.empty((9, self.Q))
+        Fn = np.empty((9, self.Q))
+        gn = np.empty_like(self.g)
+        lib().cpu_point_pass(self.h, _p(u), *self._state(dt), _p(sig), _p(Fn), _p(gn))
+        return sig, Fn, gn
+
+    def vjp_pass(self, u, dt, Pbar):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        Pbar = np.ascontiguousarray(Pbar, dtype=np.float64).reshape(9, self.Q)
+        Hb = np.zeros((9, self.Q))
+        pb = np.zeros((8, self.Q))
+        lib().cpu_vjp_pass(self.h, _p(u), *self._state(dt), _p(Pbar), _p(Hb), _p(pb))
+        return Hb, pb
+
+    def csr(self, vals):
+        import scipy.sparse as sp
+        return sp.csr_matrix((vals, self.indices, self.indptr), shape=(self.n_dofs, self.n_dofs))
+
+    def bicgstab(self, vals, b, x0, tol=1e-10, atol=1e-10, maxiter=10000):
+        x = np.array(x0, dtype=np.float64, copy=True)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        k = lib().cpu_bicgstab(self.h, _p(vals), _p(b), _p(x), tol, atol, maxiter)
+        return x, k
+
+    def time_sample(self, u, dt, krylov_iters=40, vjp=False):
+        """Seconds of one full assembly, one point pass without tangent, one BiCGSTAB
+        iteration (the mean of `krylov_iters` on a seeded right-hand side far from
+        convergence) and, with vjp, one per-point VJP pass, at dofs u."""
+        t0 = time.perf_counter()
+        res, vals, counts = self.assemble(u, dt)
+        t_asm = time.perf_counter() - t0
+        if not np.isfinite(vals).all():
+            raise RuntimeError("CPU baseline: the sampled Jacobian is not finite")
+        t0 = time.perf_counter()
+        self.point_pass(u, dt)
+        t_post = time.perf_counter() - t0
+        b = self.csr(vals) @ np.random.default_rng(0).normal(size=self.n_dofs)
+        t0 = time.perf_counter()
+        _, k = self.bicgstab(vals, b, np.zeros(self.n_dofs), tol=0.0, atol=0.0, maxiter=krylov_iters)
+        t_it = (time.perf_counter() - t0) / max(k, 1)
+        if k != krylov_iters:
+            raise RuntimeError(f"CPU baseline BiCGSTAB ran {k} of {krylov_iters} iterations")
+        info = dict(n_dofs=self.n_dofs, nnz=len(self.indices), newton_iters=int(counts[0]),
+                    residual_evals=int(counts[1]))
+        if vjp:
+            Pbar = np.random.default_rng(1).normal(size=(9, self.Q))
+            t0 = time.perf_counter()
+            self.vjp_pass(u, dt, Pbar)
+            info["vjp_s"] = time.perf_counter() - t0
+        return t_asm, t_post, t_it, info

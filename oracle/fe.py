@@ -272,8 +272,9 @@ def line_search(prob, dofs, inc):
 
 
 def solve(prob, initial_guess=None, tol=1e-6, rel_tol=1e-8, line_search_flag=False,
-          krylov=None):
-    """Row-elimination Newton; solver.py:310-437 with linear_incremental_solver :213-237."""
+          krylov=None, max_newton=200):
+    """Row-elimination Newton; solver.py:310-437 with linear_incremental_solver :213-237.
+    ``max_newton`` only guards the oracle (the reference loop has no cap) and raises."""
     dofs = np.zeros(prob.ndof) if initial_guess is None else np.asarray(initial_guess, float).ravel().copy()
     krylov = krylov or {}
 
@@ -287,6 +288,8 @@ def solve(prob, initial_guess=None, tol=1e-6, rel_tol=1e-8, line_search_flag=Fal
     rel = res_val / res0
     its = 0
     while rel > rel_tol and res_val > tol:
+        if its >= max_newton:
+            raise RuntimeError(f"oracle Newton did not converge in {max_newton} iterations (res {res_val})")
         b = -res
         x0 = np.zeros(prob.ndof)
         x0[prob.bc_dofs] = prob.bc_vals - dofs[prob.bc_dofs]

@@ -22,13 +22,15 @@ def pointlib(tmp_path_factory):
     return ctypes.CDLL(out)
 
 
-def _run(lib, table, N, Q, xm, self_block, seed):
+def _run(lib, table, N, Q, xm, self_block, seed, max_iter=250, poison=None):
     from oracle import cp_local
     rng = np.random.default_rng(seed)
     tab = slip_table(table)
     d = tab[:, 3:6] / np.linalg.norm(tab[:, 3:6], axis=1)[:, None]
     n = tab[:, 0:3] / np.linalg.norm(tab[:, 0:3], axis=1)[:, None]
     H, Fp, g, slip, q = random_states(Q, N, rng, table=table)
+    if poison is not None:
+        g[poison] = np.nan
     R = cp_local.rot_mat_from_quat(q)
     kw = dict(FCC_304, xm=xm)
     om = cp_local.Material(cp_local.schmid_tensors(tab), self_block=bool(self_block), **kw)
@@ -40,7 +42,7 @@ def _run(lib, table, N, Q, xm, self_block, seed):
                                                           mat, d, n)]
     p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
     rc = lib.cpu_point_local(N, ctypes.c_int64(Q), *[p(a) for a in ins], ctypes.c_double(1e-3), ctypes.c_double(0.5),
-                             ctypes.c_double(1e-8), 5, 250, int(self_block), p(outs["P"]), p(outs["A"]),
+                             ctypes.c_double(1e-8), 5, max_iter, int(self_block), p(outs["P"]), p(outs["A"]),
                              p(outs["Fn"]), p(outs["gn"]), p(outs["dg"]), p(outs["it"]), p(outs["st"]))
     assert rc == 0
     return outs, ref, slip
@@ -74,3 +76,27 @@ def test_planted_tangent_error_is_caught(pointlib):
     wrong = np.swapaxes(ref["A"], 1, 2)
     r = np.linalg.norm(o["A"].reshape(Q, 81) - wrong.reshape(Q, 81), axis=1) / np.linalg.norm(wrong.reshape(Q, 81), axis=1)
     assert r.max() > 1e-6
+
+
+def test_iteration_cap_sets_the_capped_status(pointlib):
+    """A point whose local Newton has not reached |r| <= tol within the cap reports
+    PT_CAPPED (bit 1); with the default cap the same points report nothing.  (The library
+    turns the bit into CPFEM_ELOCAL at the next residual norm, where the reference's
+    uncapped loop would not return.)"""
+    Q = 64
+    o, ref, _ = _run(pointlib, "slip_fcc12.txt", 12, Q, 0.02, 1, seed=5, max_iter=1)
+    needs_more = ref["iters"] > 1
+    assert needs_more.any()
+    np.testing.assert_array_equal(o["st"] & 1, needs_more.astype(np.int32))
+    np.testing.assert_array_equal(o["it"][needs_more], 1)
+
+
+def test_nonfinite_history_sets_the_nonfinite_status(pointlib):
+    """A NaN in a point's slip resistance makes its residual NaN: the loop exits as the
+    reference's `while norm > tol` does on NaN, and the point reports PT_NONFINITE (bit 2)
+    and a NaN stress; the other points are unaffected."""
+    Q = 32
+    o, ref, _ = _run(pointlib, "slip_fcc12.txt", 12, Q, 0.02, 1, seed=6, poison=(7, 3))
+    assert o["st"][7] & 2 and np.isnan(o["P"][7]).all()
+    others = np.arange(Q) != 7
+    assert np.all(o["st"][others] == 0) and np.isfinite(o["P"][others]).all()

@@ -9,7 +9,9 @@
     scale coefficients; and the L-BFGS-B loop of that driver (cpfem_amd.optimize) against the
     points the reference's calibration run logged.
 Central differences use relative step 1e-6 (1e-4 degree for angles) with the Newton
-tolerances tightened so the forward is smooth to ~1e-12.  The Krylov tolerance is 1e-12
+tolerances tightened so the forward is smooth to ~1e-10 (the local tolerance 1e-10 |r| sits just
+above the FP64 rounding floor of the residual at these stress levels; at 1e-11 some points
+stall there and hit the local iteration cap, which is an error).  The Krylov tolerance is 1e-12
 relative (atol negligible): near 1e-13 Jacobi-BiCGSTAB on these tiny systems reaches the
 rounding floor of its recurrence and can break down (rhat.q = 0) in the reference semantics
 itself, whichever way the rounding falls."""
@@ -36,7 +38,7 @@ def _poly(n_steps=3, k=2):
     quat = uniform_quaternions(2, rng)
     inds = np.array([0, 1, 1, 0, 1, 0, 0, 1])
     mat = C.Material(slip="fcc12", g0=88.0, h=500.0, t_sat=300.0, gss_a=2.0, xm=0.05, r=1.4,
-                     C11=204.6e3, C12=137.7e3, C44=126.2e3, tol=1e-11)
+                     C11=204.6e3, C12=137.7e3, C44=126.2e3, tol=1e-10)
     left = lambda p: np.isclose(p[0], 0.0, atol=1e-5)
     front = lambda p: np.isclose(p[1], 0.0, atol=1e-5)
     bottom = lambda p: np.isclose(p[2], 0.0, atol=1e-5)
@@ -117,7 +119,7 @@ def _cu_calibration():
     disps, ts = np.linspace(0, 0.025, 21), np.linspace(0, 2.5, 21)
     info = [[corner, corner, bottom, top], [0, 1, 2, 2], [zero, zero, zero, lambda p: 0.0]]
     prob = C.CrystalPlasticity(mesh, dirichlet_bc_info=info, additional_info=(np.array([[1.0, 0, 0, 0]]), np.zeros(1)),
-                               material=C.CALIB_COPPER.replace(tol=1e-11))
+                               material=C.CALIB_COPPER.replace(tol=1e-10))
     fwd_pred = C.ad_wrapper(prob, {"jax_solver": {}, **TIGHT}, ADJ)
     idx = np.arange(0, 20, 4)
 
@@ -178,7 +180,7 @@ def test_orientation_design_gradient_bcc24_dual_phase():
     inds = rng.integers(0, G, size=nc)
     phase = (inds == 3).astype(np.int64)  # one hard grain of four
     soft = C.Material(slip="bcc24", g0=100.0, h=400.0, t_sat=250.0, gss_a=2.0, xm=0.05, r=1.4,
-                      C11=231.0e3, C12=135.0e3, C44=116.0e3, tol=1e-11)
+                      C11=231.0e3, C12=135.0e3, C44=116.0e3, tol=1e-10)
     left = lambda p: np.isclose(p[0], 0.0, atol=1e-5)
     front = lambda p: np.isclose(p[1], 0.0, atol=1e-5)
     bottom = lambda p: np.isclose(p[2], 0.0, atol=1e-5)

@@ -89,7 +89,7 @@ def test_adjoint_with_direct_solver_matches_krylov():
         info = [[left, front, bottom, top], [0, 1, 2, 2], [zero, zero, zero, lambda p: 2e-3]]
         prob = C.CrystalPlasticity(mesh, dirichlet_bc_info=info,
                                    additional_info=(np.array([[1.0, 0, 0, 0]]), np.zeros(8)),
-                                   material=C.CALIB_COPPER.replace(tol=1e-11))
+                                   material=C.CALIB_COPPER.replace(tol=1e-10))
         prob.dt = 0.2
         fwd = C.ad_wrapper(prob, {**opt, "tol": 1e-9, "rel_tol": 1e-13}, dict(opt))
         alpha = torch.full((2,), 1.3, dtype=torch.float64, device="cuda", requires_grad=True)

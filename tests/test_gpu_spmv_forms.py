@@ -18,39 +18,39 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 
-def _form(n_dofs, cus):
-    """The form the library picks (krylov_spmv_form, mirrored by bench.spmv_kernel_names):
-    2 = 128-row tiles, 0 = 64-row tiles with all slot loads in flight, 1 = 64-row tiles."""
-    import importlib.util
-    import os
-    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(root, "bench.py"))
-    bench = importlib.util.module_from_spec(spec)
-    spec.loader.exec_module(bench)
-    name = bench.spmv_kernel_names(n_dofs, cus, "double")[0]
-    return 2 if name.endswith(", 2>") else 0 if "_all<" in name else 1
+def _form(prob):
+    """The SpMV form the library runs on this context (cpfem_krylov_config): 2 = 128-row tiles,
+    1 = 64-row tiles, 0 = 64-row tiles with all slot loads in flight."""
+    import ctypes
+    from cpfem_amd import _native as N
+    out = (ctypes.c_int32 * 3)()
+    N.check(prob.ctx._lib.cpfem_krylov_config(prob.ctx.h, out))
+    return int(out[0])
+
+
+def _first_system(n):
+    from cpfem_amd import configs
+    from cpfem_amd.solver import bc_x0
+    w = configs.config2(n=n)
+    prob = configs.build_problem(w)
+    prob.fes[0].update_Dirichlet_boundary_conditions(w.dirichlet_bc_info(w.disps[1]))
+    prob.dt = w.ts[1] - w.ts[0]
+    prob.set_params(prob.internal_vars)
+    dofs = torch.zeros(prob.ctx.n_dofs, dtype=torch.float64, device="cuda")
+    res, mat = prob.assemble(dofs)
+    return prob, mat, -res, bc_x0(prob, dofs)
 
 
 @pytest.mark.parametrize("n,form", [(16, 0), (26, 1), (28, 2)])
 def test_spmv_form_solves_first_newton_system(n, form):
     import cpfem_amd as C
-    from cpfem_amd import configs
-    from cpfem_amd.solver import bc_x0, krylov_solve
-    w = configs.config2(n=n)
-    prob = configs.build_problem(w)
+    from cpfem_amd.solver import krylov_solve
+    prob, mat, b, x0 = _first_system(n)
     cus = torch.cuda.get_device_properties(0).multi_processor_count
-    picked = _form(prob.ctx.n_dofs, cus)
+    picked = _form(prob)
     print(f"n={n}: {prob.ctx.n_dofs} dofs, {cus} CUs, SpMV form {picked}", flush=True)
     if cus == 256:
         assert picked == form
-    info = w.dirichlet_bc_info(w.disps[1])
-    prob.fes[0].update_Dirichlet_boundary_conditions(info)
-    prob.dt = w.ts[1] - w.ts[0]
-    prob.set_params(prob.internal_vars)
-    dofs = torch.zeros(prob.ctx.n_dofs, dtype=torch.float64, device="cuda")
-    res, mat = prob.assemble(dofs)
-    x0 = bc_x0(prob, dofs)
-    b = -res
     x = krylov_solve(prob, mat, b, x0)
     it = prob.solver_info[-1]["iterations"]
     A = C.BlockEllMatrix(prob.ctx, mat).to_scipy()
@@ -70,3 +70,30 @@ def test_spmv_form_solves_first_newton_system(n, form):
           f"|x - x64|/|x64| = {err:.2e}", flush=True)
     assert resm <= bar, (resm, bar)
     assert err < 1e-6, err
+
+
+def test_all_slots_form_reproduces_one_slot_form_bitwise():
+    """At 16^3 the all-slots form (0) and the one-slot-at-a-time form (1) run the same grid of
+    64-row tiles with the same products in the same order: forced on the same system, they give
+    bitwise-equal solutions and iteration counts (FP64 and mixed); the 128-row form (2) solves
+    the same system to the same bar."""
+    from cpfem_amd import _native as N
+    from cpfem_amd.solver import krylov_solve
+    prob, mat, b, x0 = _first_system(16)
+    L, h = prob.ctx._lib, prob.ctx.h
+    out = {}
+    for form in (0, 1, 2):
+        N.check(L.cpfem_set_spmv_form(h, form))
+        assert _form(prob) == form
+        for prec in ("fp64", "mixed"):
+            x = krylov_solve(prob, mat, b, x0, dict(precision=prec))
+            out[form, prec] = (x.cpu().numpy(), prob.solver_info[-1]["iterations"])
+    N.check(L.cpfem_set_spmv_form(h, -1))
+    for prec in ("fp64", "mixed"):
+        np.testing.assert_array_equal(out[0, prec][0], out[1, prec][0])
+        assert out[0, prec][1] == out[1, prec][1]
+        x2, x1 = out[2, prec][0], out[1, prec][0]
+        assert np.linalg.norm(x2 - x1) <= 1e-6 * np.linalg.norm(x1)
+    # form 0 is refused where tiles exceed three per CU
+    big, *_ = _first_system(28)
+    assert big.ctx._lib.cpfem_set_spmv_form(big.ctx.h, 0) == N.EBADARG

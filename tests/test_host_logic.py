@@ -52,6 +52,25 @@ def test_location_functions_vectorised_and_scalar_fallback():
     np.testing.assert_allclose(fe.vals_list[0], 1e-3)
 
 
+def test_location_function_that_reduces_over_its_point():
+    """jax-fem evaluates location and value functions point by point (vmap).  A function that
+    reduces over its point, such as np.all(np.isclose(p, c)), returns one 0-d value for the
+    whole coordinate-row array, which must not be copied to every point: here it selects
+    exactly the origin, and a value function reducing the same way gives per-point values."""
+    fe = _fe(2)
+    origin = lambda p: np.all(np.isclose(p, 0.0, atol=1e-5))
+    far = lambda p: np.all(np.isclose(p, 1.0, atol=1e-5))
+    val = lambda p: float(np.sum(p))  # 0-d for any input
+    fe.update_Dirichlet_boundary_conditions([[origin, far, lambda p: np.isclose(p[2], 1.0, atol=1e-5)], [0, 1, 2],
+                                             [lambda p: 0.0, lambda p: 3.0, val]])
+    np.testing.assert_array_equal(fe.points[fe.node_inds_list[0]], [[0.0, 0.0, 0.0]])
+    np.testing.assert_array_equal(fe.points[fe.node_inds_list[1]], [[1.0, 1.0, 1.0]])
+    np.testing.assert_allclose(fe.vals_list[1], [3.0])
+    top = fe.points[fe.node_inds_list[2]]
+    assert len(top) == 9
+    np.testing.assert_allclose(fe.vals_list[2], top.sum(axis=1))
+
+
 def test_workloads_are_deterministic_and_well_formed():
     from cpfem_amd import configs
     a, b = configs.config2(n=4), configs.config2(n=4)
