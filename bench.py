@@ -61,16 +61,17 @@ def spmv_bytes(prob, value_bytes=8):
     return 9 * value_bytes * nnzb + idx + 7 * 8 * n, nnzb
 
 
-def point_flops(n_slip, n_points, newton_iters, residual_evals):
-    """Algorithmic FP64 flops of the assembly's point kernel over n_points point updates with
-    the measured totals of local Newton iterations and residual evaluations: the reference
-    algorithm's per-unit counts (FMA = 2 flops; div, abs, sign, pow, sqrt = 1 each):
-    F_res(N) = 41 N + 350 per residual evaluation, F_jac(N) = 165 N + 2574 per local Jacobian,
-    F_LU = 648 per 9x9 LU and solve (once per iteration), F_tan = 9300 per consistent tangent,
-    F_elem = 4752 + 144 per point for the element stiffness and residual."""
+def point_flops(n_slip, tangent_points, newton_iters, residual_evals):
+    """Algorithmic FP64 flops of the assembly's point kernels (k_element_qp and the parked
+    points' k_element_qp_park) from measured totals: the reference algorithm's per-unit counts
+    (FMA = 2 flops; div, abs, sign, pow, sqrt = 1 each), F_res(N) = 41 N + 350 per residual
+    evaluation, F_jac(N) = 165 N + 2574 per local Jacobian and F_LU = 648 per 9x9 LU and solve
+    (once per iteration), F_tan = 9300 per consistent tangent (assemblies with the Jacobian
+    only; the line search's residual-only assemblies compute none).  The element stiffness
+    (F_elem = 4752 + 144 per point) is k_element_K's work, not counted here."""
     N = n_slip
-    f_res, f_jac, f_lu, f_tan, f_elem = 41 * N + 350, 165 * N + 2574, 648, 9300, 4752 + 144
-    return residual_evals * f_res + newton_iters * (f_jac + f_lu) + n_points * (f_tan + f_elem)
+    f_res, f_jac, f_lu, f_tan = 41 * N + 350, 165 * N + 2574, 648, 9300
+    return residual_evals * f_res + newton_iters * (f_jac + f_lu) + tangent_points * f_tan
 
 
 def spmv_kernel_names(n_dofs, cus, tm):
@@ -112,7 +113,7 @@ class Profile:
     def start(self):
         from cpfem_amd.problem import stream_handle
         self.N.check(self.L.cpfem_profile(self.h, 1, self.sample))
-        cnt = (ctypes.c_int64 * 2)()
+        cnt = (ctypes.c_int64 * 3)()
         self.N.check(self.L.cpfem_point_counts(self.h, stream_handle(), 1, cnt))
 
     def read(self):
@@ -121,12 +122,12 @@ class Profile:
         ms, la, ti = (ctypes.c_double * n)(), (ctypes.c_int64 * n)(), (ctypes.c_int64 * n)()
         self.N.check(self.L.cpfem_profile_read(self.h, n, ms, la, ti))
         self.N.check(self.L.cpfem_profile(self.h, 0, 1))
-        cnt = (ctypes.c_int64 * 2)()
+        cnt = (ctypes.c_int64 * 3)()
         self.N.check(self.L.cpfem_point_counts(self.h, stream_handle(), 1, cnt))
         prof = {c: dict(launches=int(la[k]), timed=int(ti[k]), avg_ms=(ms[k] / ti[k] if ti[k] else None),
                         est_total_ms=(ms[k] / ti[k] * la[k] if ti[k] else 0.0))
                 for k, c in enumerate(self.N.PROF_CLASSES)}
-        return prof, dict(newton_iters=int(cnt[0]), residual_evals=int(cnt[1]))
+        return prof, dict(newton_iters=int(cnt[0]), residual_evals=int(cnt[1]), parked_points=int(cnt[2]))
 
 
 # ------------------------------------------------------------------------------ load steps
@@ -338,6 +339,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mixed-steps", type=int, default=2, help="timed load steps in the other precision (0: none)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip both adjoint gradients")
+    ap.add_argument("--park", type=int, default=-1,
+                    help="park the point kernel's lanes when at most this many of a wave still iterate "
+                         "(0: never; -1: the library's default)")
     ap.add_argument("--history-adjoint-mesh", type=int, default=16)
     ap.add_argument("--copy-calibration", action="store_true", help="also run a plain copy kernel (counter scale)")
     ap.add_argument("--direct", action="store_true",
@@ -366,6 +370,8 @@ def main():
     log(f"[bench] workload {w.name}: {len(w.mesh.cells)} cells, {w.n_qp} points; load step {S + 1}")
     t0 = time.perf_counter()
     prob = configs.build_problem(w)
+    if args.park >= 0:
+        prob.ctx.set_point_park(args.park)
     log(f"[bench] context built in {time.perf_counter() - t0:.1f}s")
     params = prob.internal_vars
     sol = [torch.zeros(prob.fes[0].num_total_nodes, 3, dtype=torch.float64, device="cuda")]
@@ -412,7 +418,7 @@ def main():
     names = spmv_kernel_names(prob.ctx.n_dofs, cus, tm)
     tkey = f"c{args.config}_{n_mesh}"
     dom = max(("krylov_spmv", "element_qp", "element_K", "gather_matrix", "krylov_vector", "point_local"),
-              key=lambda c: kp[c]["est_total_ms"])
+              key=lambda c: kp[c]["est_total_ms"] + (kp["element_qp_park"]["est_total_ms"] if c == "element_qp" else 0))
     # Launches that move the matrix: two per BiCGSTAB iteration plus the initial residual of each
     # solve (the launches queued past convergence return at once and move nothing, so the class
     # time, sampled mean x all launches, is divided by the working launches only)
@@ -431,19 +437,24 @@ def main():
                          timing="hipExtLaunchKernelGGL start/stop events (kernel-bound) on every "
                                 f"{args.profile_sample}th launch of the class, on its launch stream")
     point_roof = None
-    qp = kp["element_qp"]
+    qp, qpk = kp["element_qp"], kp["element_qp_park"]
     if qp["avg_ms"] and qp["launches"]:
-        flops = point_flops(prob.num_slip_sys, Q * qp["launches"], work["newton_iters"], work["residual_evals"])
-        ach_tf = flops / (qp["est_total_ms"] * 1e-3) / 1e12
+        tangent_launches = assemblies  # one Jacobian assembly per Newton iteration plus the initial one
+        flops = point_flops(prob.num_slip_sys, Q * tangent_launches, work["newton_iters"], work["residual_evals"])
+        t_ms = qp["est_total_ms"] + qpk["est_total_ms"]
+        ach_tf = flops / (t_ms * 1e-3) / 1e12
         point_roof = dict(bound="fp64 valu", achieved=round(ach_tf, 3), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
                           frac=round(ach_tf / FP64_PEAK_TFLOPS, 4),
-                          kernel=f"cpfem::k_element_qp<{prob.num_slip_sys}> (local Newton + tangent per point)",
-                          launches=qp["launches"], avg_launch_ms=round(qp["avg_ms"], 4),
-                          flops_per_point=round(flops / (Q * qp["launches"]), 1),
-                          newton_iters_per_point=round(work["newton_iters"] / (Q * qp["launches"]), 3),
-                          residual_evals_per_point=round(work["residual_evals"] / (Q * qp["launches"]), 3),
+                          kernel=f"cpfem::k_element_qp<{prob.num_slip_sys}> + k_element_qp_park (local Newton + P + "
+                                 "tangent per point)",
+                          launches=qp["launches"], tangent_launches=tangent_launches,
+                          avg_launch_ms=round(t_ms / qp["launches"], 4), park_lanes=args.park,
+                          parked_points_per_launch=round(work["parked_points"] / qp["launches"], 1),
+                          flops_per_point_launch=round(flops / (Q * qp["launches"]), 1),
+                          newton_iters_per_point_launch=round(work["newton_iters"] / (Q * qp["launches"]), 3),
+                          residual_evals_per_point_launch=round(work["residual_evals"] / (Q * qp["launches"]), 3),
                           flop_model="F_res(N)=41N+350 per residual evaluation, F_jac(N)=165N+2574 + F_LU=648 per "
-                                     "iteration, F_tan=9300 + F_elem=4896 per point (reference algorithm counts)")
+                                     "iteration, F_tan=9300 per tangent (reference algorithm counts)")
     roof = spmv_roof if dom == "krylov_spmv" else (point_roof if dom == "element_qp" else
                                                     dict(bound=None, achieved=None, peak=None, unit=None, frac=None,
                                                          traffic=None, kernel=dom))

@@ -188,10 +188,19 @@ int cpfem_krylov_config(const cpfem_ctx* ctx, int32_t* out);
 int cpfem_set_spmv_form(cpfem_ctx* ctx, int32_t form);
 
 /* Work counters of the assembly's point kernel, summed over its launches since the last
-   reset: out[0] = local Newton iterations, out[1] = residual evaluations (synchronises;
-   reset != 0 clears them in stream order).  Diagnostics with no counterpart in the
-   reference: the measured inputs of the point kernel's algorithmic flop count. */
+   reset: out[0] = local Newton iterations, out[1] = residual evaluations, out[2] = parked
+   points (synchronises; reset != 0 clears them in stream order).  Diagnostics with no
+   counterpart in the reference: the measured inputs of the point kernel's algorithmic flop
+   count. */
 int cpfem_point_counts(cpfem_ctx* ctx, void* stream, int32_t reset, int64_t* out);
+
+/* Parking of the assembly point kernel's slow lanes (0..63, default 0 = none): the lanes of a
+   wave iterate their local Newton in lockstep, and once at most `lanes` of them are still
+   iterating those park their point in a queue that a second kernel finishes, one point per lane.
+   Every output is bitwise the same for any setting; only the time changes.  No counterpart
+   in the reference (whose vmapped while_loop runs every batch to its slowest point,
+   models_copper.py:247). */
+int cpfem_set_point_park(cpfem_ctx* ctx, int32_t lanes);
 
 /* y = alpha x + beta y on n_dofs entries (device). */
 int cpfem_axpby(cpfem_ctx* ctx, void* stream, double alpha, const double* x, double beta, double* y);

@@ -93,10 +93,11 @@ _SIGS = {
     "cpfem_point_counts": (ctypes.c_int, [vp, vp, ctypes.c_int32, vp]),
     "cpfem_krylov_config": (ctypes.c_int, [vp, vp]),
     "cpfem_set_spmv_form": (ctypes.c_int, [vp, ctypes.c_int32]),
+    "cpfem_set_point_park": (ctypes.c_int, [vp, ctypes.c_int32]),
 }
 
 PROF_CLASSES = ("element_qp", "element_K", "gather_matrix", "gather_residual", "krylov_spmv", "krylov_vector",
-                "point_local", "copy", "krylov_replace")
+                "point_local", "copy", "krylov_replace", "element_qp_park")
 
 EXPORTED = tuple(_SIGS)
 _lib = None

@@ -71,6 +71,18 @@ class NativeContext:
         N.check(self._lib.cpfem_geometry(self.h, sg.ctypes.data, jxw.ctypes.data))
         return sg, jxw
 
+    def set_point_park(self, lanes):
+        """Parking of the assembly point kernel's slow lanes (0 = none; cpfem_set_point_park):
+        changes the time only, never a result."""
+        N.check(self._lib.cpfem_set_point_park(self.h, int(lanes)))
+
+    def point_counts(self, reset=True):
+        """(local Newton iterations, residual evaluations, parked points) of the assembly's
+        point kernel since the last reset."""
+        out = (ctypes.c_int64 * 3)()
+        N.check(self._lib.cpfem_point_counts(self.h, stream_handle(), 1 if reset else 0, out))
+        return tuple(int(v) for v in out)
+
     def pattern(self):
         nslot = np.empty(self.n_nodes, dtype=np.int32)
         col = np.empty((self.n_slots, self.n_nodes), dtype=np.int32)

@@ -117,7 +117,12 @@ struct cpfem_ctx {
   // device: element work buffers
   double *d_wP = nullptr, *d_wA = nullptr, *d_KE = nullptr, *d_RE = nullptr, *d_sig = nullptr;
   int32_t* d_status = nullptr;
-  unsigned long long* d_counts = nullptr;  // [0] local Newton iterations, [1] residual evaluations
+  unsigned long long* d_counts = nullptr;  // [0] local Newton iterations, [1] residual evaluations, [2] parked points
+  // parking of the assembly point kernel's slow lanes (allocated on first use)
+  int32_t *d_park_idx = nullptr, *d_park_it = nullptr;
+  double* d_park_s = nullptr;
+  uint32_t* d_park_ctr = nullptr;
+  int park_lanes = 0;  // 0: no parking
   // reductions
   int nb = 0;
   double* d_partial = nullptr;
@@ -388,6 +393,15 @@ static int ensure_matrix_buffers(cpfem_ctx* c) {
   return CPFEM_OK;
 }
 
+static int ensure_park(cpfem_ctx* c) {
+  int rc;
+  if (c->d_park_ctr) return CPFEM_OK;
+  if ((rc = dalloc(&c->d_park_idx, (size_t)c->n_qp)) || (rc = dalloc(&c->d_park_it, (size_t)c->n_qp)) ||
+      (rc = dalloc(&c->d_park_s, (size_t)6 * c->n_qp)) || (rc = dalloc(&c->d_park_ctr, 2)))
+    return rc;
+  return CPFEM_OK;
+}
+
 static int ensure_krylov(cpfem_ctx* c, bool mixed) {
   int rc;
   const int64_t n = c->n_dofs;
@@ -530,7 +544,7 @@ int cpfem_create(const CpfemMesh* mesh, const CpfemSlip* slip, cpfem_ctx** out) 
   c->nb = std::max(c->nb, 1);
   if ((rc = dalloc(&c->d_isbc, c->n_dofs)) || (rc = dalloc(&c->d_nobc, c->n_dofs)) || (rc = dalloc(&c->d_ubar, c->n_dofs)) ||
       (rc = dalloc(&c->d_wP, (size_t)9 * c->n_qp)) || (rc = dalloc(&c->d_RE, (size_t)24 * c->n_cells)) ||
-      (rc = dalloc(&c->d_sig, (size_t)9 * c->n_qp)) || (rc = dalloc(&c->d_status, 1)) || (rc = dalloc(&c->d_counts, 2)) ||
+      (rc = dalloc(&c->d_sig, (size_t)9 * c->n_qp)) || (rc = dalloc(&c->d_status, 1)) || (rc = dalloc(&c->d_counts, 3)) ||
       (rc = dalloc(&c->d_partial, (size_t)4 * std::max(c->nb, nbq))) || (rc = dalloc(&c->d_scalar, 4)) ||
       (rc = dalloc(&c->d_counter, 4))) {
     const std::string e = g_err;
@@ -540,11 +554,11 @@ int cpfem_create(const CpfemMesh* mesh, const CpfemSlip* slip, cpfem_ctx** out) 
   }
   if (hipMemset(c->d_isbc, 0, c->n_dofs) != hipSuccess || hipMemset(c->d_nobc, 0, c->n_dofs) != hipSuccess || hipMemset(c->d_ubar, 0, 8 * c->n_dofs) != hipSuccess ||
       hipMemset(c->d_status, 0, 4) != hipSuccess || hipMemset(c->d_counter, 0, 16) != hipSuccess ||
-      hipMemset(c->d_counts, 0, 16) != hipSuccess ||
+      hipMemset(c->d_counts, 0, 24) != hipSuccess ||
       hipHostMalloc((void**)&c->h_ksc, sizeof(KrylovScalars)) != hipSuccess ||
       hipHostMalloc((void**)&c->h_dbl, 8 * sizeof(double)) != hipSuccess ||
       hipHostMalloc((void**)&c->h_int, 8 * sizeof(int32_t)) != hipSuccess ||
-      hipHostMalloc((void**)&c->h_cnt, 2 * sizeof(unsigned long long)) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_cnt, 3 * sizeof(unsigned long long)) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev[0], hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev[1], hipEventDisableTiming) != hipSuccess) {
     cpfem_destroy(c);
@@ -563,6 +577,7 @@ void cpfem_destroy(cpfem_ctx* c) {
   dfree(c->d_cptr); dfree(c->d_clist); dfree(c->d_nptr); dfree(c->d_nlist);
   dfree(c->d_isbc); dfree(c->d_nobc); dfree(c->d_ubar);
   dfree(c->d_wP); dfree(c->d_wA); dfree(c->d_KE); dfree(c->d_RE); dfree(c->d_sig); dfree(c->d_status); dfree(c->d_counts);
+  dfree(c->d_park_idx); dfree(c->d_park_it); dfree(c->d_park_s); dfree(c->d_park_ctr);
   dfree(c->d_partial); dfree(c->d_scalar); dfree(c->d_counter);
   dfree(c->d_ksc);
   dfree(c->d_kpartial);
@@ -667,12 +682,26 @@ int cpfem_assemble(cpfem_ctx* c, void* stream, const double* dofs, const CpfemSt
   const SolveCfg cfg = solve_cfg(params);
   double* wA = mat ? c->d_wA : nullptr;
   Prof* pf = &c->prof;
+  PointPark pk{c->d_park_idx, c->d_park_s, c->d_park_it, c->d_park_ctr, c->n_qp, c->park_lanes};
+  if (pk.lanes > 0) {
+    if ((rc = ensure_park(c))) return rc;
+    pk = PointPark{c->d_park_idx, c->d_park_s, c->d_park_it, c->d_park_ctr, c->n_qp, c->park_lanes};
+    launch_zero_words(pk.ctr, 2, st);
+  }
   int sl = prof_begin(pf, PC_ELEMENT_QP, st);
   if (c->n_slip == 12)
-    launch_element_qp<12>(G, dofs, sv, mv, c->tab12, cfg, c->d_wP, wA, iters, c->d_status, c->d_counts, st);
+    launch_element_qp<12>(G, dofs, sv, mv, c->tab12, cfg, c->d_wP, wA, iters, c->d_status, c->d_counts, pk, st);
   else
-    launch_element_qp<24>(G, dofs, sv, mv, c->tab24, cfg, c->d_wP, wA, iters, c->d_status, c->d_counts, st);
+    launch_element_qp<24>(G, dofs, sv, mv, c->tab24, cfg, c->d_wP, wA, iters, c->d_status, c->d_counts, pk, st);
   prof_end(pf, sl, st);
+  if (pk.lanes > 0) {
+    sl = prof_begin(pf, PC_ELEMENT_PARK, st);
+    if (c->n_slip == 12)
+      launch_element_qp_park<12>(G, dofs, sv, mv, c->tab12, cfg, c->d_wP, wA, iters, c->d_status, c->d_counts, pk, st);
+    else
+      launch_element_qp_park<24>(G, dofs, sv, mv, c->tab24, cfg, c->d_wP, wA, iters, c->d_status, c->d_counts, pk, st);
+    prof_end(pf, sl, st);
+  }
   sl = prof_begin(pf, PC_ELEMENT_K, st);
   launch_element_K(G, c->d_wP, wA, c->d_KE, c->d_RE, st);
   prof_end(pf, sl, st);
@@ -808,6 +837,13 @@ int cpfem_norm(cpfem_ctx* c, void* stream, const double* v, double* out, int32_t
   return CPFEM_OK;
 }
 
+int cpfem_set_point_park(cpfem_ctx* c, int32_t lanes) {
+  if (!c) return fail(CPFEM_EBADARG, "null context");
+  if (lanes < 0 || lanes > 63) return fail(CPFEM_EBADARG, "park lanes must be in 0..63");
+  c->park_lanes = lanes;
+  return CPFEM_OK;
+}
+
 int cpfem_krylov_config(const cpfem_ctx* c, int32_t* out) {
   if (!c || !out) return fail(CPFEM_EBADARG, "null argument");
   out[0] = c->krylov_form();
@@ -828,12 +864,13 @@ int cpfem_set_spmv_form(cpfem_ctx* c, int32_t form) {
 int cpfem_point_counts(cpfem_ctx* c, void* stream, int32_t reset, int64_t* out) {
   if (!c || !out) return fail(CPFEM_EBADARG, "null argument");
   hipStream_t st = (hipStream_t)stream;
-  HIPCHK(hipMemcpyAsync(c->h_cnt, c->d_counts, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-  if (reset) launch_zero_words(reinterpret_cast<uint32_t*>(c->d_counts), 4, st);
+  HIPCHK(hipMemcpyAsync(c->h_cnt, c->d_counts, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+  if (reset) launch_zero_words(reinterpret_cast<uint32_t*>(c->d_counts), 6, st);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(st));
   out[0] = (int64_t)c->h_cnt[0];
   out[1] = (int64_t)c->h_cnt[1];
+  out[2] = (int64_t)c->h_cnt[2];
   return CPFEM_OK;
 }
 

@@ -57,10 +57,29 @@ struct BellView {
   int n_slots;
 };
 
+// Parking of the assembly point kernel's slow lanes.  The lanes of a wave iterate their local
+// Newton in lockstep; once at most `lanes` of them are still iterating, those park their
+// point (index, stress iterate, iteration count) in this queue and the wave moves on.  A
+// second kernel then finishes the parked points densely, one queue entry per lane.
+// lanes = 0: no parking.
+struct PointPark {
+  int32_t* idx;   // [cap] parked point index
+  double* s;      // [6][cap] its stress iterate (Voigt)
+  int32_t* it;    // [cap] iterations done
+  uint32_t* ctr;  // [0] parked points, [1] unused
+  int64_t cap;
+  int lanes;
+};
+
 template <int N>
 void launch_element_qp(const Geom& G, const double* u, const StateView& sv, const MatView& mv, const SlipTable<N>& tab,
                        const SolveCfg& cfg, double* wP, double* wA, int32_t* iters, int32_t* status,
-                       unsigned long long* counts, hipStream_t st);
+                       unsigned long long* counts, const PointPark& park, hipStream_t st);
+// The second kernel over the parked points (grid sized to the device, chunks taken atomically).
+template <int N>
+void launch_element_qp_park(const Geom& G, const double* u, const StateView& sv, const MatView& mv, const SlipTable<N>& tab,
+                            const SolveCfg& cfg, double* wP, double* wA, int32_t* iters, int32_t* status,
+                            unsigned long long* counts, const PointPark& park, hipStream_t st);
 template <int N>
 void launch_local(int64_t n_qp, const double* Hd, const Geom& G, const double* u, const StateView& sv, const MatView& mv,
                   const SlipTable<N>& tab, const SolveCfg& cfg, int self_block, double* P, double* A, double* sig,
@@ -110,6 +129,7 @@ enum ProfClass {
   PC_LOCAL,           // point kernel for stress / history (k_local)
   PC_COPY,            // plain copy (bandwidth calibration)
   PC_KREPLACE,        // residual replacement of the mixed form (k_kreplace)
+  PC_ELEMENT_PARK,    // parked points of the assembly's point kernel (k_element_qp_park)
   PC_COUNT
 };
 

@@ -5,6 +5,8 @@
 // Upstream element semantics (jax-fem, used through solver.py:392 and restated by
 // models_copper.py:277-278,315): u_grad_q = sum_a u_a (x) dN_a(q);
 // r_e[a,i] = sum_q P_q,ij dN_a,j JxW_q; K_e[a i, b k] = sum_q dN_a,j A_q[ij,kl] dN_b,l JxW_q.
+#include <algorithm>
+
 #include "cp_internal.hpp"
 
 namespace cpfem {
@@ -64,56 +66,160 @@ __device__ __forceinline__ uint32_t wave_count(uint32_t v) {
   return v;
 }
 
+// Point q's inputs: F = I + u_grad from the dofs, Fp_inv_old and R, and the solver set-up
+// (rotated stiffness, 1/g_old in LDS).
+template <int N>
+__device__ __forceinline__ void start_point(const Geom& G, const double* __restrict__ u, const StateView& sv,
+                                            const MatView& mv, int64_t q, double* sh, Point<N>& pt, double* F,
+                                            double* Fpi) {
+  double H[9], R[9];
+  ugrad_from_dofs(G, u, q, H);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) F[i] = H[i] + ((i % 4) == 0 ? 1.0 : 0.0);
+  load_point(sv, q, Fpi, R);
+  const PointMat m = load_mat(mv, q);
+  pt.setup(F, Fpi, R, sv.g + q, sv.ld, m, sh + threadIdx.x, sh + N * QBS + threadIdx.x, QBS);
+}
+
+// Outputs of a converged point: JxW P, JxW dP/du_grad, iterations, status bits.
+template <int N>
+__device__ __forceinline__ void finish_point(const Geom& G, const SlipTable<N>& tab, int64_t q, Point<N>& pt,
+                                             const double* F, const double* Fpi, const double* s, int it, int32_t st,
+                                             double* __restrict__ wP, double* __restrict__ wA,
+                                             int32_t* __restrict__ iters, int32_t* __restrict__ status) {
+  const double w = jxw_at(G, q);
+  double P[9];
+  pk1<N>(pt, F, Fpi, s, P);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) wP[i * G.n_qp + q] = w * P[i];
+  if (wA) {
+    tangent<N>(pt, tab, F, Fpi, s, [&](int m9, const double* dP) {
+#pragma unroll
+      for (int i = 0; i < 9; ++i) wA[(int64_t)(9 * i + m9) * G.n_qp + q] = w * dP[i];
+    });
+  }
+  if (iters) iters[q] = it;
+  if (st) atomicOr(status, st);
+}
+
+__device__ __forceinline__ void add_counts(unsigned long long* counts, uint32_t n_it, uint32_t n_ev) {
+  if (!counts) return;
+  n_it = wave_count(n_it);
+  n_ev = wave_count(n_ev);
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&counts[0], (unsigned long long)n_it);
+    atomicAdd(&counts[1], (unsigned long long)n_ev);
+  }
+}
+
+constexpr double DBL_MAXV = 1.7976931348623157e308;
+
 // ---------------------------------------------------------------------------------
 // Quadrature-point kernel of the assembly: H from dofs, local Newton, JxW-weighted P (SoA
-// [9][Q]) and, if wA, the JxW-weighted consistent tangent stored per point ([Q][81]: the
-// element kernel's 8 threads of a cell then read whole cache lines together).  counts
-// (optional): [0] += local Newton iterations, [1] += residual evaluations over the launch
-// (one atomic per wave), the measured inputs of the algorithmic flop count.
+// [9][Q]) and, if wA, the JxW-weighted consistent tangent, also structure-of-arrays
+// ([81][Q]: each of the 81 stores of a wave writes 512 contiguous bytes).
+// The lanes of a wave run their Newton iterations in lockstep; a wave whose points need very
+// different iteration counts would otherwise run until its slowest lane converges.  With
+// parking (pk.lanes > 0), once at most pk.lanes lanes of the wave are still iterating, those
+// lanes store their point's iterate in the park queue and the wave finishes the others;
+// k_element_qp_park completes the parked points.  The iterates, and so every output, are the
+// same either way: a parked point resumes from its stored stress with the residual
+// re-evaluated there.  counts (optional): [0] += local Newton iterations, [1] += residual
+// evaluations (one atomic per wave), the measured inputs of the algorithmic flop count.
 template <int N>
 __global__ void __launch_bounds__(QBS) k_element_qp(Geom G, const double* __restrict__ u, StateView sv, MatView mv,
                                                    SlipTable<N> tab, SolveCfg cfg, double* __restrict__ wP,
                                                    double* __restrict__ wA, int32_t* __restrict__ iters,
-                                                   int32_t* __restrict__ status, unsigned long long* __restrict__ counts) {
+                                                   int32_t* __restrict__ status, unsigned long long* __restrict__ counts,
+                                                   PointPark pk) {
   __shared__ double sh[2 * N * QBS];
   const int64_t q = (int64_t)blockIdx.x * QBS + threadIdx.x;
-  uint32_t n_it = 0, n_ev = 0;
-  if (q < G.n_qp) {
-    double H[9], F[9], Fpi[9], R[9];
-    ugrad_from_dofs(G, u, q, H);
+  const bool valid = q < G.n_qp;
+  Point<N> pt;
+  double F[9], Fpi[9], s[6];
+  int it = 0;
+  int32_t st = 0;
 #pragma unroll
-    for (int i = 0; i < 9; ++i) F[i] = H[i] + ((i % 4) == 0 ? 1.0 : 0.0);
-    load_point(sv, q, Fpi, R);
-    const PointMat m = load_mat(mv, q);
-    Point<N> pt;
-    pt.setup(F, Fpi, R, sv.g + q, sv.ld, m, sh + threadIdx.x, sh + N * QBS + threadIdx.x, QBS);
-    double s[6];
-    int32_t st = 0;
-    const int it = pt.solve(tab, cfg.tol, cfg.max_sub_step, cfg.max_iter, s, &st);
-    const double w = jxw_at(G, q);
-    double P[9];
-    pk1<N>(pt, F, Fpi, s, P);
-#pragma unroll
-    for (int i = 0; i < 9; ++i) wP[i * G.n_qp + q] = w * P[i];
-    if (wA) {
-      tangent<N>(pt, tab, F, Fpi, s, [&](int m9, const double* dP) {
-#pragma unroll
-        for (int i = 0; i < 9; ++i) wA[q * 81 + 9 * i + m9] = w * dP[i];
-      });
+  for (int k = 0; k < 6; ++k) s[k] = 0.0;
+  pt.nev = 0;
+  pt.nr = 0.0;
+  if (valid) {
+    start_point<N>(G, u, sv, mv, q, sh, pt, F, Fpi);
+    pt.eval(tab, s);
+  }
+  bool act = valid && pt.nr > cfg.tol;
+  for (;;) {  // wave-uniform exits: every lane sees the same ballot
+    const int nact = __popcll(__ballot(act));
+    if (nact == 0 || nact <= pk.lanes) break;
+    if (act) {
+      if (it >= cfg.max_iter) {
+        st |= PT_CAPPED;
+        act = false;
+      } else {
+        pt.newton_step(tab, cfg.max_sub_step, s);
+        ++it;
+        act = pt.nr > cfg.tol;
+      }
     }
-    if (iters) iters[q] = it;
-    if (st) atomicOr(status, st);
+  }
+  const unsigned long long park = __ballot(act);
+  if (park) {  // park the lanes still iterating (one queue reservation per wave)
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll(park) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&pk.ctr[0], (uint32_t)__popcll(park));
+    base = __shfl(base, leader, 64);
+    if (act) {
+      const int64_t k = (int64_t)base + __popcll(park & ((1ull << lane) - 1ull));
+      pk.idx[k] = (int32_t)q;
+      pk.it[k] = it;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) pk.s[j * pk.cap + k] = s[j];
+    }
+  }
+  uint32_t n_it = 0;
+  if (valid && !act) {
+    if (!(fabs(pt.nr) <= DBL_MAXV)) st |= PT_NONFINITE;
+    finish_point<N>(G, tab, q, pt, F, Fpi, s, it, st, wP, wA, iters, status);
+    n_it = (uint32_t)it;
+  }
+  add_counts(counts, n_it, valid ? (uint32_t)pt.nev : 0u);
+}
+
+// The parked points of k_element_qp, one queue entry per lane: each point's stress iterate is
+// restored, its residual re-evaluated there and its Newton iterations continued to
+// convergence.  The grid covers the longest possible queue; blocks past the queue's length
+// (final when this kernel starts: k_element_qp ran before it on the same stream) exit at once,
+// so only as many blocks as parked points run, dispatched as earlier blocks retire.  (Looping
+// each wave over the queue instead costs this kernel 64 spilled registers per lane.)
+template <int N>
+__global__ void __launch_bounds__(QBS) k_element_qp_park(Geom G, const double* __restrict__ u, StateView sv, MatView mv,
+                                                        SlipTable<N> tab, SolveCfg cfg, double* __restrict__ wP,
+                                                        double* __restrict__ wA, int32_t* __restrict__ iters,
+                                                        int32_t* __restrict__ status,
+                                                        unsigned long long* __restrict__ counts, PointPark pk) {
+  __shared__ double sh[2 * N * QBS];
+  const uint32_t total = __hip_atomic_load(&pk.ctr[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if ((int64_t)blockIdx.x * QBS >= (int64_t)total) return;  // uniform over the block
+  if (counts && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&counts[2], (unsigned long long)total);
+  const int64_t k = (int64_t)blockIdx.x * QBS + threadIdx.x;
+  uint32_t n_it = 0, n_ev = 0;
+  if (k < (int64_t)total) {
+    const int64_t q = pk.idx[k];
+    Point<N> pt;
+    double F[9], Fpi[9], s[6];
+    start_point<N>(G, u, sv, mv, q, sh, pt, F, Fpi);
+#pragma unroll
+    for (int j = 0; j < 6; ++j) s[j] = pk.s[j * pk.cap + k];
+    pt.eval(tab, s);
+    int it = pk.it[k];
+    int32_t st = 0;
+    pt.iterate(tab, cfg.tol, cfg.max_sub_step, cfg.max_iter, s, &it, &st);
+    finish_point<N>(G, tab, q, pt, F, Fpi, s, it, st, wP, wA, iters, status);
     n_it = (uint32_t)it;
     n_ev = (uint32_t)pt.nev;
   }
-  if (counts) {
-    n_it = wave_count(n_it);
-    n_ev = wave_count(n_ev);
-    if ((threadIdx.x & 63) == 0) {
-      atomicAdd(&counts[0], (unsigned long long)n_it);
-      atomicAdd(&counts[1], (unsigned long long)n_ev);
-    }
-  }
+  add_counts(counts, n_it, n_ev);
 }
 
 // ---------------------------------------------------------------------------------
@@ -145,7 +251,7 @@ __global__ void __launch_bounds__(256) k_element_K(Geom G, const double* __restr
       for (int j = 0; j < 3; ++j) s += wP[(3 * i + j) * G.n_qp + qp] * ga[j];
       re[i] += s;
     }
-    const double* Aq = wA + qp * 81;
+    const double* Aq = wA + qp;  // [81][Q]: component m at Aq[m * Q]
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       double Z[9];  // Z[k][l] = sum_j ga_j A[(i j),(k l)]
@@ -153,7 +259,7 @@ __global__ void __launch_bounds__(256) k_element_K(Geom G, const double* __restr
       for (int kl = 0; kl < 9; ++kl) {
         double s = 0.0;
 #pragma unroll
-        for (int j = 0; j < 3; ++j) s += ga[j] * Aq[9 * (3 * i + j) + kl];
+        for (int j = 0; j < 3; ++j) s += ga[j] * Aq[(int64_t)(9 * (3 * i + j) + kl) * G.n_qp];
         Z[kl] = s;
       }
 #pragma unroll
@@ -343,10 +449,19 @@ __global__ void k_bc_x0(int64_t n, const uint8_t* __restrict__ is_bc, const doub
 template <int N>
 void launch_element_qp(const Geom& G, const double* u, const StateView& sv, const MatView& mv, const SlipTable<N>& tab,
                        const SolveCfg& cfg, double* wP, double* wA, int32_t* iters, int32_t* status,
-                       unsigned long long* counts, hipStream_t st) {
+                       unsigned long long* counts, const PointPark& park, hipStream_t st) {
   const int64_t nb = (G.n_qp + QBS - 1) / QBS;
   hipLaunchKernelGGL(k_element_qp<N>, dim3((unsigned)nb), dim3(QBS), 0, st, G, u, sv, mv, tab, cfg, wP, wA, iters, status,
-                     counts);
+                     counts, park);
+}
+
+template <int N>
+void launch_element_qp_park(const Geom& G, const double* u, const StateView& sv, const MatView& mv, const SlipTable<N>& tab,
+                            const SolveCfg& cfg, double* wP, double* wA, int32_t* iters, int32_t* status,
+                            unsigned long long* counts, const PointPark& park, hipStream_t st) {
+  const int64_t nb = (park.cap + QBS - 1) / QBS;
+  hipLaunchKernelGGL(k_element_qp_park<N>, dim3((unsigned)nb), dim3(QBS), 0, st, G, u, sv, mv, tab, cfg, wP, wA, iters,
+                     status, counts, park);
 }
 
 template <int N>
@@ -408,7 +523,10 @@ void launch_cell_average(const Geom& G, const double* sig, double* out, hipStrea
 #define CPFEM_INST(N)                                                                                                      \
   template void launch_element_qp<N>(const Geom&, const double*, const StateView&, const MatView&, const SlipTable<N>&,    \
                                      const SolveCfg&, double*, double*, int32_t*, int32_t*, unsigned long long*,           \
-                                     hipStream_t);                                                                         \
+                                     const PointPark&, hipStream_t);                                                       \
+  template void launch_element_qp_park<N>(const Geom&, const double*, const StateView&, const MatView&,                    \
+                                          const SlipTable<N>&, const SolveCfg&, double*, double*, int32_t*, int32_t*,      \
+                                          unsigned long long*, const PointPark&, hipStream_t);                             \
   template void launch_local<N>(int64_t, const double*, const Geom&, const double*, const StateView&, const MatView&,      \
                                 const SlipTable<N>&, const SolveCfg&, int, double*, double*, double*, const StateOut&,     \
                                 int32_t*, int32_t*, hipStream_t);

@@ -351,6 +351,47 @@ struct Point {
     }
   }
 
+  // One local Newton iteration from the last evaluated point s (models_copper.py:218-247):
+  // d = -J^-1 r, then trials s + lam d for lam = 1, 1/2, ... until |r_trial| < |r| or
+  // max_sub_step trials; the last trial is accepted (s <- trial, r and nr are its residual).
+  CPFEM_HD void newton_step(const SlipTable<N>& tab, int max_sub_step, double* s) {
+    double dlt[6];
+    {
+      double J[36];
+      int piv[6];
+      jac(tab, J);
+      lu6(J, piv);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) dlt[q] = -r[q];
+      lu6_solve(J, piv, dlt);
+    }
+    const double nr0 = nr;
+    double lam = 1.0;
+    int k = 0;
+    double trial[6];
+    do {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) trial[q] = s[q] + lam * dlt[q];
+      eval(tab, trial);
+      lam *= 0.5;
+      ++k;
+    } while (nr >= nr0 && k < max_sub_step);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) s[q] = trial[q];
+  }
+
+  // Iterations from the current evaluated point while |r| > tol, counted in *it, up to it_end;
+  // sets PT_CAPPED if it_end is reached unconverged and PT_NONFINITE on a non-finite |r|.
+  CPFEM_HD void iterate(const SlipTable<N>& tab, double tol, int max_sub_step, int it_end, double* s, int* it,
+                        int32_t* stat) {
+    while (nr > tol) {
+      if (*it >= it_end) { *stat |= PT_CAPPED; break; }
+      newton_step(tab, max_sub_step, s);
+      ++*it;
+    }
+    if (!(fabs(nr) <= 1.7976931348623157e308)) *stat |= PT_NONFINITE;
+  }
+
   // Local Newton from S = 0 with cut-half backtracking (models_copper.py:204-249).
   // Returns the iteration count; status bits in *stat; s holds the converged stress.
   CPFEM_HD int solve(const SlipTable<N>& tab, double tol, int max_sub_step, int max_iter, double* s, int32_t* stat) {
@@ -358,34 +399,7 @@ struct Point {
     for (int q = 0; q < 6; ++q) s[q] = 0.0;
     eval(tab, s);
     int it = 0;
-    while (nr > tol) {
-      if (it >= max_iter) { *stat |= PT_CAPPED; break; }
-      double dlt[6];
-      {
-        double J[36];
-        int piv[6];
-        jac(tab, J);
-        lu6(J, piv);
-#pragma unroll
-        for (int q = 0; q < 6; ++q) dlt[q] = -r[q];
-        lu6_solve(J, piv, dlt);
-      }
-      const double nr0 = nr;
-      double lam = 1.0;
-      int k = 0;
-      double trial[6];
-      do {
-#pragma unroll
-        for (int q = 0; q < 6; ++q) trial[q] = s[q] + lam * dlt[q];
-        eval(tab, trial);
-        lam *= 0.5;
-        ++k;
-      } while (nr >= nr0 && k < max_sub_step);
-#pragma unroll
-      for (int q = 0; q < 6; ++q) s[q] = trial[q];
-      ++it;
-    }
-    if (!(fabs(nr) <= 1.7976931348623157e308)) *stat |= PT_NONFINITE;
+    iterate(tab, tol, max_sub_step, max_iter, s, &it, stat);
     return it;
   }
 };
