@@ -174,11 +174,13 @@ def step_adjoint(prob, w, index, szz_ref, precision):
     loss = (szz - target) ** 2
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    n_info = len(prob.solver_info)
     loss.backward()
     torch.cuda.synchronize()
     t2 = time.perf_counter()
-    return dict(forward_s=t1 - t0, backward_s=t2 - t1, grad_s=t2 - t0, szz=float(szz), loss=float(loss),
-                grad=theta.grad.detach().cpu().numpy())
+    adj_iters = sum(max(d["iterations"], 0) for d in prob.solver_info[n_info:])
+    return dict(forward_s=t1 - t0, backward_s=t2 - t1, grad_s=t2 - t0, szz=float(szz.detach()), loss=float(loss.detach()),
+                backward_krylov_iters=adj_iters, grad=theta.grad.detach().cpu().numpy())
 
 
 def history_adjoint(n=16, k=10, fd=True, mixed=True):
@@ -250,23 +252,27 @@ def history_adjoint(n=16, k=10, fd=True, mixed=True):
 
 
 # ------------------------------------------------------------------------------ CPU baseline
-def cpu_baseline(prob, w, params, sol, index, per_step, krylov_iters=40):
+def cpu_baseline(prob, w, params, sol, index, per_step, krylov_iters=40, adjoint_krylov_iters=None):
     """The C++/OpenMP restatement of the reference algorithm (oracle/cpu) on all host cores,
     on a bounded sample of the timed load step: one full assembly (every point's local Newton
-    and tangent, element stiffness, CSR gather) at the converged state and `krylov_iters`
-    Jacobi-BiCGSTAB iterations on the assembled CSR matrix, scaled by the GPU run's per-step
-    counts of assemblies (plus one point pass each for the cell stress and the history
-    update) and Krylov iterations."""
-    import numpy as np
+    and tangent, element stiffness, CSR gather), one residual-only assembly (the line search's),
+    one point pass without tangent and `krylov_iters` Jacobi-BiCGSTAB iterations on the
+    assembled CSR matrix at the converged state, scaled by the GPU run's per-step counts of
+    Jacobian assemblies, residual-only assemblies, point passes (cell stress, history update)
+    and Krylov iterations.  With adjoint_krylov_iters (the GPU backward's transposed-solve
+    iterations of the load-step adjoint) also one per-point VJP pass, and the adjoint gradient
+    is estimated as the load step plus its backward: two VJP passes, one Jacobian assembly at
+    the converged state and the transposed solve."""
     from oracle.cpu import cpu_ref
-    import torch
     cores = cpu_ref.threads()
     mat = cpu_ref.Model.from_problem(prob, params, w)
     u = sol[0].reshape(-1).detach().cpu().numpy()
     # the timed step's Dirichlet values (the history is the snapshot's)
     prob.dt = float(w.ts[index + 1] - w.ts[index])
-    t_asm, t_post, t_it, info = mat.time_sample(u, float(prob.dt), krylov_iters)
-    step_s = (per_step["assemblies"] * t_asm + 2 * t_post + per_step["krylov_iters"] * t_it)
+    t_asm, t_post, t_it, info = mat.time_sample(u, float(prob.dt), krylov_iters, vjp=adjoint_krylov_iters is not None)
+    t_res = info["res_s"]
+    step_s = (per_step["assemblies"] * t_asm + per_step["residual_assemblies"] * t_res + 2 * t_post
+              + per_step["krylov_iters"] * t_it)
     Q = w.n_qp
     model = platform.processor() or platform.machine()
     try:
@@ -276,16 +282,25 @@ def cpu_baseline(prob, w, params, sol, index, per_step, krylov_iters=40):
                 break
     except OSError:
         pass
-    return dict(value=Q / step_s, unit="quad-pt load-step updates/s", cores=cores, kind="port",
-                sample=(f"C++/OpenMP restatement of the reference algorithm (oracle/cpu), {cores} threads on "
-                        f"{os.cpu_count()} logical CPUs ({model}): one full assembly of all {Q} points at the timed "
-                        f"step's converged state ({t_asm:.2f} s: local Newton + tangent + element stiffness + CSR "
-                        f"gather; {info['newton_iters'] / Q:.2f} local iterations per point), one point pass without "
-                        f"tangent ({t_post:.2f} s) and {krylov_iters} Jacobi-BiCGSTAB iterations on the "
-                        f"{info['n_dofs']}-dof CSR matrix ({t_it * 1e3:.1f} ms/iteration), scaled by the GPU run's "
-                        f"per-step counts ({per_step['assemblies']:.1f} assemblies, 2 point passes, "
-                        f"{per_step['krylov_iters']:.0f} Krylov iterations): est. {step_s:.1f} s per load step"),
-                load_step_s=step_s)
+    out = dict(value=Q / step_s, unit="quad-pt load-step updates/s", cores=cores, kind="port",
+               sample=(f"C++/OpenMP restatement of the reference algorithm (oracle/cpu), {cores} threads on "
+                       f"{os.cpu_count()} logical CPUs ({model}): one full assembly of all {Q} points at the timed "
+                       f"step's converged state ({t_asm:.2f} s: local Newton + tangent + element stiffness + CSR "
+                       f"gather; {info['newton_iters'] / Q:.2f} local iterations per point), one residual-only "
+                       f"assembly ({t_res:.2f} s), one point pass without tangent ({t_post:.2f} s) and {krylov_iters} "
+                       f"Jacobi-BiCGSTAB iterations on the {info['n_dofs']}-dof CSR matrix ({t_it * 1e3:.1f} "
+                       f"ms/iteration), scaled by the GPU run's per-step counts ({per_step['assemblies']:.1f} Jacobian "
+                       f"assemblies, {per_step['residual_assemblies']:.1f} residual-only assemblies, 2 point passes, "
+                       f"{per_step['krylov_iters']:.0f} Krylov iterations): est. {step_s:.1f} s per load step"),
+               load_step_s=step_s)
+    if adjoint_krylov_iters is not None:
+        t_vjp = info["vjp_s"]
+        back = 2 * t_vjp + t_asm + adjoint_krylov_iters * t_it
+        out["adjoint_grad_s"] = step_s + back
+        out["adjoint_sample"] = (f"load step ({step_s:.1f} s) + backward: 2 VJP passes ({t_vjp:.2f} s each), one "
+                                 f"Jacobian assembly, {adjoint_krylov_iters} transposed-solve iterations "
+                                 f"= est. {step_s + back:.1f} s")
+    return out
 
 
 def direct_bench(n, reps):
@@ -406,8 +421,11 @@ def main():
     kry = [d["iterations"] for d in prob.solver_info]
     log(f"[bench] timed {K} steps in {T:.3f}s ({T / K:.3f} s/step); newton {counts['newton'] / K:.1f}/step; "
         "profile: " + ", ".join(f"{c}={kp[c]['est_total_ms']:.1f}ms/{kp[c]['launches']}" for c in kp))
-    assemblies = counts["newton"] + K  # one assembly per Newton iteration plus the initial one
-    per_step = dict(assemblies=assemblies / K, krylov_iters=sum(max(k, 0) for k in kry) / K)
+    assemblies = counts["newton"] + K  # one Jacobian assembly per Newton iteration plus the initial one
+    # every assembly launches the point kernel once; the others are the line search's residuals
+    res_assemblies = max(kp["element_qp"]["launches"] - assemblies, 0)
+    per_step = dict(assemblies=assemblies / K, residual_assemblies=res_assemblies / K,
+                    krylov_iters=sum(max(k, 0) for k in kry) / K)
 
     Q = w.n_qp
     value = Q * K / T
@@ -519,8 +537,15 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (C++/OpenMP restatement) ...")
-        cpu = cpu_baseline(prob, w, runner.params, [final_sol[0]], S, per_step)
+        adj = detail.get("adjoint_load_step")
+        cpu = cpu_baseline(prob, w, runner.params, [final_sol[0]], S, per_step,
+                           adjoint_krylov_iters=adj["backward_krylov_iters"] if adj else None)
         detail["speedup_vs_cpu_load_step"] = cpu["load_step_s"] / (T / K)
+        if adj:
+            detail["speedup_vs_cpu_adjoint_grad"] = dict(fp64=cpu["adjoint_grad_s"] / adj["grad_s"],
+                                                         mixed=cpu["adjoint_grad_s"] / adj["mixed"]["grad_s"])
+        if "compare" in detail and args.precision == "fp64":
+            detail["speedup_vs_cpu_load_step_mixed"] = cpu["load_step_s"] / detail["compare"]["other_load_step_s"]
         log(f"[bench] cpu {cpu['value']:.1f} {cpu['unit']} ({cpu['load_step_s']:.1f} s per load step)")
 
     line = {

@@ -16,7 +16,12 @@ ARCH = os.environ.get("CPFEM_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-I", INCLUDE,
          # packed FP32 instructions stay off: some of their forms have given wrong results on
          # MI355X with no error
-         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         # multiply-adds are fused per source expression only (not by the back end across
+         # statements), so an inline function compiles to the same arithmetic in every kernel
+         # it is inlined into: a parked point finished by the second assembly kernel gets
+         # bitwise the results the first kernel would have produced
+         "-ffp-contract=on"]
 
 
 def _hipcc():

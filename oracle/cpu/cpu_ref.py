@@ -161,14 +161,19 @@ class Model:
         return x, k
 
     def time_sample(self, u, dt, krylov_iters=40, vjp=False):
-        """Seconds of one full assembly, one point pass without tangent, one BiCGSTAB
-        iteration (the mean of `krylov_iters` on a seeded right-hand side far from
-        convergence) and, with vjp, one per-point VJP pass, at dofs u."""
+        """Seconds of one full assembly, one residual-only assembly (the line search's), one
+        point pass without tangent, one BiCGSTAB iteration (the mean of `krylov_iters` on a
+        seeded right-hand side far from convergence) and, with vjp, one per-point VJP pass,
+        at dofs u.  Returns (t_asm, t_post, t_it, info) with info["res_s"] the residual-only
+        assembly."""
         t0 = time.perf_counter()
         res, vals, counts = self.assemble(u, dt)
         t_asm = time.perf_counter() - t0
         if not np.isfinite(vals).all():
             raise RuntimeError("CPU baseline: the sampled Jacobian is not finite")
+        t0 = time.perf_counter()
+        self.assemble(u, dt, want_matrix=False)
+        t_res = time.perf_counter() - t0
         t0 = time.perf_counter()
         self.point_pass(u, dt)
         t_post = time.perf_counter() - t0
@@ -179,7 +184,7 @@ class Model:
         if k != krylov_iters:
             raise RuntimeError(f"CPU baseline BiCGSTAB ran {k} of {krylov_iters} iterations")
         info = dict(n_dofs=self.n_dofs, nnz=len(self.indices), newton_iters=int(counts[0]),
-                    residual_evals=int(counts[1]))
+                    residual_evals=int(counts[1]), res_s=t_res)
         if vjp:
             Pbar = np.random.default_rng(1).normal(size=(9, self.Q))
             t0 = time.perf_counter()
