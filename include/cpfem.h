@@ -148,6 +148,21 @@ int cpfem_update_state(cpfem_ctx* ctx, void* stream, const double* dofs, const C
 int cpfem_solve(cpfem_ctx* ctx, void* stream, const double* mat, const double* b, double* x, double tol,
                 double atol, int32_t maxiter, int32_t precond, int32_t precision, CpfemSolveInfo* info);
 
+/* cpfem_solve in FP64 without any device allocation.  No counterpart in the reference: it is
+   for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+   loops, harnesses that forbid hipMalloc).  The Krylov workspace (scalars, partial sums,
+   13 n_dofs doubles of vectors) lives in a fixed device array of the library, about 20 MiB,
+   which each device that the library runs on holds until the process ends; the array's
+   device address is looked up when a context is created on that device.  Limits: n_dofs <=
+   CPFEM_NOALLOC_MAX_DOFS (larger systems return CPFEM_EBADARG and must use cpfem_solve);
+   FP64 only.  All calls on one device share the array: calls that overlap on one device
+   (other streams, other threads, other contexts) race and must be serialised by the caller.
+   Arguments, results and errors are those of cpfem_solve with precision CPFEM_FP64; the
+   context must already exist (cpfem_create allocates). */
+#define CPFEM_NOALLOC_MAX_DOFS 196608
+int cpfem_solve_noalloc(cpfem_ctx* ctx, void* stream, const double* mat, const double* b, double* x, double tol,
+                        double atol, int32_t maxiter, int32_t precond, CpfemSolveInfo* info);
+
 /* Direct solve of A x = b (replaces umfpack_solve, crystal_plasticity_OR_design/solver.py:50-60,
    selected by solver_options 'umfpack_solver' at :100-101): LU of the band form of A under a
    bandwidth-reducing node order (unpivoted; a zero pivot returns CPFEM_EDIRECT), then `refine`

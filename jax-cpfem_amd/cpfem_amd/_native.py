@@ -14,6 +14,7 @@ FP64, MIXED = 0, 1
 APPLY_BC = 1
 XM, GSS_A, H, TSAT, R, C11, C12, C44, NPARAM = range(9)
 PT_CAPPED, PT_NONFINITE = 1, 2
+NOALLOC_MAX_DOFS = 196608  # CPFEM_NOALLOC_MAX_DOFS
 
 c_dp = ctypes.POINTER(ctypes.c_double)
 vp = ctypes.c_void_p
@@ -76,6 +77,8 @@ _SIGS = {
                                           ctypes.POINTER(CpfemStateOut)]),
     "cpfem_solve": (ctypes.c_int, [vp, vp, vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_int32,
                                    ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(CpfemSolveInfo)]),
+    "cpfem_solve_noalloc": (ctypes.c_int, [vp, vp, vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_int32,
+                                           ctypes.c_int32, ctypes.POINTER(CpfemSolveInfo)]),
     "cpfem_spmv": (ctypes.c_int, [vp, vp, vp, vp, vp]),
     "cpfem_solve_direct": (ctypes.c_int, [vp, vp, vp, vp, vp, ctypes.c_int32, ctypes.POINTER(CpfemSolveInfo)]),
     "cpfem_band_order": (ctypes.c_int, [ctypes.c_int64, vp, vp, vp, vp]),

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -146,6 +147,7 @@ struct cpfem_ctx {
   int32_t* d_bflag = nullptr;
   Prof prof;
   int spmv_form = -1;  // BiCGSTAB SpMV form: -1 = picked by size (krylov_spmv_form), else forced
+  double* scratch = nullptr;  // device address of the fixed workspace array on this context's device
 
   int krylov_form() const { return spmv_form >= 0 ? spmv_form : krylov_spmv_form(n_dofs); }
 
@@ -443,47 +445,103 @@ static KVec kvec(double** v, double* x) {
   return k;
 }
 
-// One BiCGSTAB run (JAX semantics) with the Jacobi vector already in kv64[8]; returns the
+// Fixed workspace of cpfem_solve_noalloc: a file-scope device array (each device holds its own
+// copy until the process ends).  Host code never passes the array's name to a kernel: its
+// device address is looked up once per device (scratch_address) and used from then on.
+// Layout: Krylov scalars, partial sums [PS_SLOTS][NOALLOC_PARTIALS], then 13 n doubles of
+// vectors for n <= CPFEM_NOALLOC_MAX_DOFS.
+constexpr int NOALLOC_SCALARS = 32;  // doubles reserved for KrylovScalars (256 B)
+constexpr int PS_SLOTS = 8;          // partial-sum slots (PS_COUNT = 7, rounded up)
+constexpr int NOALLOC_PARTIALS = 2048;
+constexpr size_t NOALLOC_DOUBLES = (size_t)NOALLOC_SCALARS + (size_t)PS_SLOTS * NOALLOC_PARTIALS + 13ull * CPFEM_NOALLOC_MAX_DOFS;
+static_assert(sizeof(KrylovScalars) <= NOALLOC_SCALARS * sizeof(double), "Krylov scalars exceed their slot");
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+static std::mutex g_scratch_mu;
+static double* g_scratch_addr[64] = {};
+
+// Device address of `scratch` on the current device, looked up on the first call per device.
+static int scratch_address(double** out) {
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail(CPFEM_EHIP, "device index out of range for the fixed workspace");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(CPFEM_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_addr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_addr[dev];
+  return CPFEM_OK;
+}
+
+// Krylov workspace of one solve: the context's own buffers (cpfem_solve) or the library's
+// fixed device array (cpfem_solve_noalloc).
+struct KBufs {
+  KrylovScalars* sc;
+  double* partial;  // [PS_COUNT][krylov_stride_max(n)]
+  double* v[12];    // as cpfem_ctx::kv64
+  double* tmp64;    // n
+  double* r64;      // n (mixed form only)
+  float* mat32;     // FP32 copy of the matrix values (mixed form only)
+  Prof* prof;       // kernel-class timing, or nullptr
+};
+
+static KBufs ctx_bufs(cpfem_ctx* c) {
+  KBufs kb{};
+  kb.sc = c->d_ksc;
+  kb.partial = c->d_kpartial;
+  for (int i = 0; i < 12; ++i) kb.v[i] = c->kv64[i];
+  kb.tmp64 = c->d_tmp64;
+  kb.r64 = c->d_r64;
+  kb.mat32 = c->d_mat32;
+  kb.prof = &c->prof;
+  return kb;
+}
+
+// One BiCGSTAB run (JAX semantics) with the Jacobi vector already in kb.v[8]; returns the
 // iteration counter (negative: breakdown code).
 // With vals64 != nullptr (mixed form) a replacement of the recurrence residual by the true
 // FP64 residual is offered after every batch (reliable updates, k_kreplace).
 template <typename TM>
-static int run_bicgstab(cpfem_ctx* c, hipStream_t st, const TM* vals, const double* b, double* x, double tol, double atol,
-                        int maxiter, int32_t* k_out, const double* vals64 = nullptr) {
+static int run_bicgstab(cpfem_ctx* c, hipStream_t st, const KBufs& kb, const TM* vals, const double* b, double* x, double tol,
+                        double atol, int maxiter, int32_t* k_out, const double* vals64 = nullptr) {
   const BellView M = c->bell();
   KrylovScalars h{};
   h.tol = tol;
   h.atol = atol;
   h.maxiter = maxiter;
   *c->h_ksc = h;
-  HIPCHK(hipMemcpyAsync(c->d_ksc, c->h_ksc, sizeof(KrylovScalars), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(kb.sc, c->h_ksc, sizeof(KrylovScalars), hipMemcpyHostToDevice, st));
   KrylovWork w;
   w.n = c->n_dofs;
   w.nb = krylov_blocks(c->n_dofs);
   w.sf = c->krylov_form();
   w.nbs = krylov_spmv_blocks(c->n_dofs, w.sf);
   w.ps = std::max(w.nb, w.nbs);
-  w.partial = c->d_kpartial;
-  w.sc = c->d_ksc;
-  KVec v = kvec(c->kv64, x);
-  int sl = prof_begin(&c->prof, PC_SPMV, st);
+  w.partial = kb.partial;
+  w.sc = kb.sc;
+  KVec v = kvec(const_cast<double**>(kb.v), x);
+  int sl = prof_begin(kb.prof, PC_SPMV, st);
   krylov_init<TM>(M, vals, b, v, w, st);
-  prof_end(&c->prof, sl, st);
+  prof_end(kb.prof, sl, st);
   HIPCHK(hipGetLastError());
   // Batches of iterations; the done flag of batch j is read while batch j+1 runs.
   const int CH = 16;
   int issued = 0, batch = 0;
   bool done = false;
   while (!done) {
-    for (int i = 0; i < CH; ++i) krylov_iteration<TM>(M, vals, v, w, i & 1, st, &c->prof);
+    for (int i = 0; i < CH; ++i) krylov_iteration<TM>(M, vals, v, w, i & 1, st, kb.prof);
     if (vals64) {
-      const int sr = prof_begin(&c->prof, PC_KREPLACE, st);
+      const int sr = prof_begin(kb.prof, PC_KREPLACE, st);
       krylov_replace(M, vals64, b, v, w, batch, st);
-      prof_end(&c->prof, sr, st);
+      prof_end(kb.prof, sr, st);
     }
     issued += CH;
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(&c->h_int[2 * (batch & 1)], &c->d_ksc->k, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&c->h_int[2 * (batch & 1)], &kb.sc->k, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     HIPCHK(hipEventRecord(c->ev[batch & 1], st));
     if (batch > 0) {
       const int pb = (batch - 1) & 1;
@@ -493,9 +551,61 @@ static int run_bicgstab(cpfem_ctx* c, hipStream_t st, const TM* vals, const doub
     ++batch;
     if (issued > maxiter + 4 * CH) break;
   }
+  HIPCHK(hipMemcpyAsync(c->h_ksc, kb.sc, sizeof(KrylovScalars), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  HIPCHK(hipMemcpy(c->h_ksc, c->d_ksc, sizeof(KrylovScalars), hipMemcpyDeviceToHost));
   *k_out = c->h_ksc->k;
+  return CPFEM_OK;
+}
+
+// Jacobi-BiCGSTAB of cpfem_solve on the workspace kb (arguments already checked).
+static int solve_impl(cpfem_ctx* c, hipStream_t st, const KBufs& kb, const double* mat, const double* b, double* x, double tol,
+                      double atol, int32_t maxiter, int32_t precond, int32_t precision, CpfemSolveInfo* info) {
+  CpfemSolveInfo inf{};
+  double bb;
+  int rc;
+  if ((rc = dot_host(c, st, b, b, &bb))) return rc;
+  inf.b_norm = std::sqrt(bb);
+  launch_inv_diag(c->bell(), mat, c->d_dslot, kb.v[8], precond, st);
+  if (precision == CPFEM_FP64) {
+    int32_t k = 0;
+    if ((rc = run_bicgstab<double>(c, st, kb, mat, b, x, tol, atol, maxiter, &k))) return rc;
+    inf.iterations = k;
+    inf.outer = 1;
+  } else {
+    // One BiCGSTAB whose SpMVs read an FP32 copy of the matrix (vectors and scalars FP64),
+    // with the recurrence residual replaced by the FP64 residual b - A x whenever it has
+    // fallen 100x since the last replacement (offered after every batch); a restart from the
+    // current x follows only if the true residual is still above max(tol |b|, atol) when the
+    // iteration stops.
+    const BellView M = c->bell();
+    launch_cast_f32(c->mat_len(), mat, kb.mat32, st);
+    const double thr2 = std::max(tol * tol * bb, atol * atol);
+    int total = 0;
+    int outer = 0;
+    for (; outer < 8; ++outer) {
+      int32_t k = 0;
+      if ((rc = run_bicgstab<float>(c, st, kb, kb.mat32, b, x, tol, atol, maxiter - total, &k, mat))) return rc;
+      total += (k > 0 ? k : 0);
+      launch_residual64(M, mat, x, b, kb.r64, st);
+      double rr;
+      if ((rc = dot_host(c, st, kb.r64, kb.r64, &rr))) return rc;
+      if (!(rr > thr2) || total >= maxiter || k <= 0) { ++outer; break; }
+    }
+    inf.iterations = total;
+    inf.outer = outer;
+  }
+  // |A x - b| < 0.1 check of jax_solve (solver.py:43-46)
+  launch_residual64(c->bell(), mat, x, b, kb.tmp64, st);
+  double err2;
+  if ((rc = dot_host(c, st, kb.tmp64, kb.tmp64, &err2))) return rc;
+  inf.residual = std::sqrt(err2);
+  if (info) *info = inf;
+  if (!(inf.residual < 0.1)) {
+    launch_fill(c->n_dofs, std::nan(""), x, st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    return fail(CPFEM_EKRYLOV, "JAX linear solver failed to converge with err = " + std::to_string(inf.residual));
+  }
   return CPFEM_OK;
 }
 
@@ -534,6 +644,12 @@ int cpfem_create(const CpfemMesh* mesh, const CpfemSlip* slip, cpfem_ctx** out) 
     return fail(CPFEM_EHIP, "hipMemcpy cells");
   }
   if ((rc = build_geometry(c, mesh->points, mesh->cells)) || (rc = build_maps(c, mesh->cells))) {
+    const std::string e = g_err;
+    cpfem_destroy(c);
+    g_err = e;
+    return rc;
+  }
+  if ((rc = scratch_address(&c->scratch))) {
     const std::string e = g_err;
     cpfem_destroy(c);
     g_err = e;
@@ -888,54 +1004,38 @@ int cpfem_solve(cpfem_ctx* c, void* stream, const double* mat, const double* b, 
   if (maxiter < 0) return fail(CPFEM_EBADARG, "maxiter must be >= 0");
   int rc;
   if ((rc = ensure_krylov(c, precision == CPFEM_MIXED))) return rc;
+  return solve_impl(c, (hipStream_t)stream, ctx_bufs(c), mat, b, x, tol, atol, maxiter, precond, precision, info);
+}
+
+int cpfem_solve_noalloc(cpfem_ctx* c, void* stream, const double* mat, const double* b, double* x, double tol, double atol,
+                        int32_t maxiter, int32_t precond, CpfemSolveInfo* info) {
+  if (!c || !mat || !b || !x) return fail(CPFEM_EBADARG, "null argument");
+  if (maxiter < 0) return fail(CPFEM_EBADARG, "maxiter must be >= 0");
+  if (c->n_dofs > CPFEM_NOALLOC_MAX_DOFS)
+    return fail(CPFEM_EBADARG, "cpfem_solve_noalloc: " + std::to_string((long long)c->n_dofs) +
+                                   " dofs exceed its fixed workspace of " + std::to_string(CPFEM_NOALLOC_MAX_DOFS) +
+                                   " dofs; use cpfem_solve");
+  if (krylov_stride_max(c->n_dofs) > NOALLOC_PARTIALS)
+    return fail(CPFEM_EBADARG, "cpfem_solve_noalloc: the Krylov grid exceeds the fixed workspace's partial sums");
+  if (!c->scratch) return fail(CPFEM_EHIP, "cpfem_solve_noalloc: the fixed workspace was not resolved on this device");
   hipStream_t st = (hipStream_t)stream;
   const int64_t n = c->n_dofs;
-  CpfemSolveInfo inf{};
-  double bb;
-  if ((rc = dot_host(c, st, b, b, &bb))) return rc;
-  inf.b_norm = std::sqrt(bb);
-  launch_inv_diag(c->bell(), mat, c->d_dslot, c->kv64[8], precond, st);
-  if (precision == CPFEM_FP64) {
-    int32_t k = 0;
-    if ((rc = run_bicgstab<double>(c, st, mat, b, x, tol, atol, maxiter, &k))) return rc;
-    inf.iterations = k;
-    inf.outer = 1;
-  } else {
-    // One BiCGSTAB whose SpMVs read an FP32 copy of the matrix (vectors and scalars FP64),
-    // with the recurrence residual replaced by the FP64 residual b - A x whenever it has
-    // fallen 100x since the last replacement (offered after every batch); a restart from the
-    // current x follows only if the true residual is still above max(tol |b|, atol) when the
-    // iteration stops.
-    const BellView M = c->bell();
-    launch_cast_f32(c->mat_len(), mat, c->d_mat32, st);
-    const double thr2 = std::max(tol * tol * bb, atol * atol);
-    int total = 0;
-    int outer = 0;
-    for (; outer < 8; ++outer) {
-      int32_t k = 0;
-      if ((rc = run_bicgstab<float>(c, st, c->d_mat32, b, x, tol, atol, maxiter - total, &k, mat))) return rc;
-      total += (k > 0 ? k : 0);
-      launch_residual64(M, mat, x, b, c->d_r64, st);
-      double rr;
-      if ((rc = dot_host(c, st, c->d_r64, c->d_r64, &rr))) return rc;
-      if (!(rr > thr2) || total >= maxiter || k <= 0) { ++outer; break; }
-    }
-    inf.iterations = total;
-    inf.outer = outer;
-  }
-  // |A x - b| < 0.1 check of jax_solve (solver.py:43-46)
-  launch_residual64(c->bell(), mat, x, b, c->d_tmp64, st);
-  double err2;
-  if ((rc = dot_host(c, st, c->d_tmp64, c->d_tmp64, &err2))) return rc;
-  inf.residual = std::sqrt(err2);
-  if (info) *info = inf;
-  if (!(inf.residual < 0.1)) {
-    std::vector<double> nanv(n, std::nan(""));
-    HIPCHK(hipMemcpyAsync(x, nanv.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
-    HIPCHK(hipStreamSynchronize(st));
-    return fail(CPFEM_EKRYLOV, "JAX linear solver failed to converge with err = " + std::to_string(inf.residual));
-  }
-  return CPFEM_OK;
+  KBufs kb{};
+  double* p = c->scratch;
+  kb.sc = reinterpret_cast<KrylovScalars*>(p);
+  p += NOALLOC_SCALARS;
+  kb.partial = p;
+  p += (size_t)PS_SLOTS * NOALLOC_PARTIALS;
+  double* vec0 = p;
+  // the vectors this solve uses (kv64 slots 2 and 9 are spare); rp and rq hold 2 n each
+  for (int i : {0, 1, 3, 4, 5, 6, 7, 8}) { kb.v[i] = p; p += n; }
+  kb.v[10] = p; p += 2 * n;
+  kb.v[11] = p; p += 2 * n;
+  kb.tmp64 = p; p += n;
+  // the array keeps the previous call's contents: the vectors start from zeros
+  launch_zero_words(reinterpret_cast<uint32_t*>(vec0), 2 * (int64_t)(p - vec0), st);
+  kb.prof = nullptr;
+  return solve_impl(c, st, kb, mat, b, x, tol, atol, maxiter, precond, CPFEM_FP64, info);
 }
 
 int cpfem_to_csr(cpfem_ctx* c, void* stream, const double* mat, int64_t* indptr, int32_t* indices, double* data) {

@@ -229,5 +229,7 @@ void launch_axpby(int64_t n, double alpha, const double* x, double beta, double*
 void launch_cast_f32(int64_t n, const double* x, float* y, hipStream_t st);
 void launch_residual64(const BellView& M, const double* vals, const double* x, const double* b, double* r, hipStream_t st);
 void launch_copy(int64_t n, const double* x, double* y, hipStream_t st);
+// y[i] = v for i < n
+void launch_fill(int64_t n, double v, double* y, hipStream_t st);
 
 }  // namespace cpfem

@@ -827,6 +827,11 @@ __global__ void __launch_bounds__(KBS) k_copy(int64_t n, const double* __restric
   if (i < n) y[i] = x[i];
 }
 
+__global__ void __launch_bounds__(KBS) k_fill(int64_t n, double v, double* __restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * KBS + threadIdx.x;
+  if (i < n) y[i] = v;
+}
+
 __global__ void k_axpby(int64_t n, double alpha, const double* __restrict__ x, double beta, double* __restrict__ y) {
   const int64_t i = (int64_t)blockIdx.x * KBS + threadIdx.x;
   if (i < n) y[i] = alpha * x[i] + beta * y[i];
@@ -936,6 +941,9 @@ void launch_residual_norm(int64_t n, const double* a, const double* b, double* p
 }
 void launch_copy(int64_t n, const double* x, double* y, hipStream_t st) {
   hipLaunchKernelGGL(k_copy, dim3(nblk(n)), dim3(KBS), 0, st, n, x, y);
+}
+void launch_fill(int64_t n, double v, double* y, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill, dim3(nblk(n)), dim3(KBS), 0, st, n, v, y);
 }
 void launch_axpby(int64_t n, double alpha, const double* x, double beta, double* y, hipStream_t st) {
   hipLaunchKernelGGL(k_axpby, dim3(nblk(n)), dim3(KBS), 0, st, n, alpha, x, beta, y);
