@@ -74,16 +74,13 @@ def point_flops(n_slip, tangent_points, newton_iters, residual_evals):
     return residual_evals * f_res + newton_iters * (f_jac + f_lu) + tangent_points * f_tan
 
 
-def spmv_kernel_names(n_dofs, cus, tm):
-    """Names of the BiCGSTAB SpMV kernel pair the library runs for n_dofs rows on a device
-    with `cus` compute units (the library's krylov_spmv_form): 128-row tiles when there are
-    two per CU, else 64-row tiles, with every slot's loads in flight when all of them fit
-    three blocks per CU."""
-    if -(-n_dofs // 128) >= 2 * cus:
-        return [f"cpfem::k_kpq<{tm}, 2>", f"cpfem::k_kst<{tm}, 2>"]
-    if -(-n_dofs // 64) <= 3 * cus:
-        return [f"cpfem::k_kpq_all<{tm}>", f"cpfem::k_kst_all<{tm}>"]
-    return [f"cpfem::k_kpq<{tm}, 1>", f"cpfem::k_kst<{tm}, 1>"]
+def spmv_kernel_names(form, tm):
+    """Names of the BiCGSTAB SpMV kernel pair of an SpMV form (as the library reports it,
+    cpfem_krylov_config): 2 = 128-row tiles, 1 = 64-row tiles, 0 = 64-row tiles with every
+    slot's loads in flight."""
+    return {2: [f"cpfem::k_kpq<{tm}, 2>", f"cpfem::k_kst<{tm}, 2>"],
+            1: [f"cpfem::k_kpq<{tm}, 1>", f"cpfem::k_kst<{tm}, 1>"],
+            0: [f"cpfem::k_kpq_all<{tm}>", f"cpfem::k_kst_all<{tm}>"]}[form]
 
 
 def measured_traffic(workload_key, precision, kernels):
@@ -357,6 +354,8 @@ def main():
     ap.add_argument("--park", type=int, default=-1,
                     help="park the point kernel's lanes when at most this many of a wave still iterate "
                          "(0: never; -1: the library's default)")
+    ap.add_argument("--spmv-form", type=int, default=-1,
+                    help="force the BiCGSTAB SpMV form (0-2; -1: the library's choice by size)")
     ap.add_argument("--history-adjoint-mesh", type=int, default=16)
     ap.add_argument("--copy-calibration", action="store_true", help="also run a plain copy kernel (counter scale)")
     ap.add_argument("--direct", action="store_true",
@@ -387,6 +386,8 @@ def main():
     prob = configs.build_problem(w)
     if args.park >= 0:
         prob.ctx.set_point_park(args.park)
+    if args.spmv_form >= 0:
+        prob.ctx.set_spmv_form(args.spmv_form)
     log(f"[bench] context built in {time.perf_counter() - t0:.1f}s")
     params = prob.internal_vars
     sol = [torch.zeros(prob.fes[0].num_total_nodes, 3, dtype=torch.float64, device="cuda")]
@@ -433,7 +434,8 @@ def main():
     sb, nnzb = spmv_bytes(prob, value_bytes=vb)
     cus = torch.cuda.get_device_properties(0).multi_processor_count
     tm = "float" if vb == 4 else "double"
-    names = spmv_kernel_names(prob.ctx.n_dofs, cus, tm)
+    form = prob.ctx.krylov_config()[0]
+    names = spmv_kernel_names(form, tm)
     tkey = f"c{args.config}_{n_mesh}"
     dom = max(("krylov_spmv", "element_qp", "element_K", "gather_matrix", "krylov_vector", "point_local"),
               key=lambda c: kp[c]["est_total_ms"] + (kp["element_qp_park"]["est_total_ms"] if c == "element_qp" else 0))
@@ -480,7 +482,7 @@ def main():
     detail = dict(
         kernel_profile=kp, dominant=dom, newton_per_step=counts["newton"] / K, krylov_per_solve=kry[: 40],
         per_step=per_step, point_work=work, sigma_bar_zz=szz[0], sigma_bar_zz_spread=max(szz) - min(szz),
-        nnz_blocks=nnzb, n_dofs=prob.ctx.n_dofs, spmv_bytes_per_launch=sb, load_step_s=T / K,
+        nnz_blocks=nnzb, n_dofs=prob.ctx.n_dofs, krylov_form=form, spmv_bytes_per_launch=sb, load_step_s=T / K,
         spmv_roofline=spmv_roof, point_roofline=point_roof)
 
     if args.mixed_steps > 0:

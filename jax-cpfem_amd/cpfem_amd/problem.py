@@ -76,6 +76,17 @@ class NativeContext:
         changes the time only, never a result."""
         N.check(self._lib.cpfem_set_point_park(self.h, int(lanes)))
 
+    def set_spmv_form(self, form):
+        """Forces the BiCGSTAB SpMV form (-1: by system size; cpfem_set_spmv_form): changes the
+        time, and the rounding of the dot products, never the algorithm."""
+        N.check(self._lib.cpfem_set_spmv_form(self.h, int(form)))
+
+    def krylov_config(self):
+        """(SpMV form, SpMV blocks, vector-kernel blocks) of this context's BiCGSTAB."""
+        out = (ctypes.c_int32 * 3)()
+        N.check(self._lib.cpfem_krylov_config(self.h, out))
+        return tuple(int(v) for v in out)
+
     def point_counts(self, reset=True):
         """(local Newton iterations, residual evaluations, parked points) of the assembly's
         point kernel since the last reset."""

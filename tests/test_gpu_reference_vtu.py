@@ -31,16 +31,17 @@ Checks: per-step cell means of sigma_xx, sigma_yy, sigma_zz and von Mises on the
 the field's largest magnitude.  Tolerances, with the largest errors observed on MI355X, are in
 TOL.  The ta and dp runs agree to the float32 storage.
 
-On steel304 the per-cell bar is looser, and it is set by the reference's own stopping rule,
-not by the kernels: on this 0.016 mm cube the nodal forces are ~1e-4, so the absolute Newton
-tolerance 1e-6 (solver.py:385, 407) stops every load step at a residual of up to ~1e-2 of the
-force scale (in step 1 after a single Newton iteration, whose iterate is a BiCGSTAB solution
-at a 1e-10 relative recurrence residual).  Two correct runs that differ only in rounding stop
-at different points inside that band and carry the difference into the history, so per cell
-they agree to that band's width, not better: 2e-2 of the field's largest magnitude.  Observed:
-7.5e-4 per cell with one compilation of the kernels, 7.3e-3 with another that fuses
-multiply-adds per source expression only (same algorithm, different last bits); cell means
-average the differences out (1.3e-6).
+On steel304 the per-cell fields are checked against a bar measured in the same test, because
+the reference's own stopping rule leaves them open: on this 0.016 mm cube the nodal forces are
+~1e-4, so the absolute Newton tolerance 1e-6 (solver.py:385, 407) stops every load step at a
+residual of up to ~1e-2 of the force scale (in step 1 after a single Newton iteration, whose
+iterate is a BiCGSTAB solution at a 1e-10 relative recurrence residual).  Two correct runs that
+differ only in rounding stop at different points inside that band and carry the difference
+into the history (observed per cell: 7e-4 in the first two stored steps, 6e-3 and 7e-3 in the
+last two; nodal u up to 6e-2).  The test therefore replays the run a second time with the
+driven displacement scaled by 1 + 2^-50 and requires the difference to the reference, per
+stored step, to stay within 10x the spread of these two runs of this side (or within 1e-3);
+cell means average the differences out (1.3e-6) and keep their fixed bar.
 
 With the single-corner x/y pins of the steel304 and ta runs, rotation about z through the
 pinned node is a zero-energy mode: the reference's BiCGSTAB and this one each leave a
@@ -58,11 +59,11 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 # case -> (tolerance on means / |mean szz|, on per-cell sigma / max|sigma|, on nodal u / max|u|);
-# largest observed: steel304 1.3e-6, 7.3e-3, 4.0e-4 (per-cell bar: see above); ta 4.5e-8, 2.5e-7, 8.8e-8;
+# largest observed: steel304 1.3e-6, 7.3e-3, 6.0e-2 (per-cell bars: see above); ta 4.5e-8, 2.5e-7, 8.8e-8;
 # dp 2.5e-9, 5.1e-8, 4.8e-8 before this side's own float32 VTU rounding (each float32 storage
 # rounds a per-cell value by up to 6e-8 of it, so the two stored copies may differ by 1.2e-7 more)
 TOL = {
-    "steel304": (2e-6, 2e-2, 2e-2),
+    "steel304": (2e-6, 1e-3, 1e-3),
     "ta": (2e-7, 1e-6, 5e-7),
     "dp": (1e-8, 3e-7, 3e-7),
 }
@@ -111,15 +112,18 @@ def _setup(case):
     return prob, info, get_top, disps, ts, steps, opts, ref
 
 
-@pytest.mark.parametrize("case", ["steel304", "ta", "dp"])
-def test_reference_run_reproduces_stored_fields(case, tmp_path):
+def _replay(case, out_dir, disp_scale=1.0):
+    """The driver loop of the stored run: per-step cell means of sigma_xx, yy, zz and von
+    Mises, and per-cell sigma / nodal u of the stored steps read back from this side's VTU."""
     import cpfem_amd as C
     from cpfem_amd.utils import cell_stress_fields, save_sol
     from test_host_vtu import _decode
     prob, info, get_top, disps, ts, steps, opts, ref = _setup(case)
+    disps = disps * disp_scale
     sol = [torch.zeros(prob.fes[0].num_total_nodes, 3, dtype=torch.float64, device="cuda")]
     params = prob.internal_vars
     means, full, usol = [], {}, {}
+    os.makedirs(out_dir, exist_ok=True)
     for i in range(steps):
         prob.dt = ts[i + 1] - ts[i]
         info[-1][-1] = get_top(disps[i + 1])
@@ -133,24 +137,24 @@ def test_reference_run_reproduces_stored_fields(case, tmp_path):
         means.append([xx.mean(), yy.mean(), zz.mean(), vm.mean()])
         if i in ref["full_steps"]:
             # through the drivers' VTU output and back, as the reference's fields were stored
-            path = str(tmp_path / f"u_{i:03d}.vtu")
+            path = os.path.join(str(out_dir), f"u_{i:03d}.vtu")
             save_sol(prob.fes[0], sol[0], path, cell_infos=list(cell_stress_fields(s).items()))
             a = _decode(path)
             full[i] = np.stack([a["sigma_xx"], a["sigma_yy"], a["sigma_zz"]], axis=1).astype(np.float64)
             usol[i] = a["sol"].reshape(-1, 3).astype(np.float64)
             np.testing.assert_allclose(a["von_Mises_stress"], vm, rtol=1e-6)
         params = prob.update_int_vars_gp(sol[0], params)
-    t_mean, t_cell, t_u = TOL[case]
-    pts = prob.fes[0].points
-    means = np.array(means)
-    rm = ref["mean_sigma"]
-    # sigma_zz and von Mises carry the load; the lateral means are small, compare on the zz scale
-    e_mean = (np.abs(means - rm) / np.abs(rm[:, 2:3])).max(axis=1)
+    return np.array(means), full, usol, prob.fes[0].points, ref
+
+
+def _field_errors(case, full, usol, pts, ref_sigma, ref_u, steps_list):
+    """Per stored step: max per-cell sigma difference on the max|sigma| scale and max nodal u
+    difference on the max|u| scale (u with the free rotation about z removed, see above)."""
     e_cell, e_u = [], []
-    for k, i in enumerate(ref["full_steps"]):
-        rf = ref["full_sigma"][k].astype(np.float64)
+    for k, i in enumerate(steps_list):
+        rf = ref_sigma[k].astype(np.float64)
         e_cell.append(np.abs(full[int(i)] - rf).max() / np.abs(rf).max())
-        ru = ref["full_sol"][k].astype(np.float64)
+        ru = ref_u[k].astype(np.float64)
         d = usol[int(i)] - ru
         if case != "dp":
             # corner x/y pins leave rotation about the z axis through the pinned node free:
@@ -159,8 +163,32 @@ def test_reference_run_reproduces_stored_fields(case, tmp_path):
             mode = np.stack([-pts[:, 1], pts[:, 0], np.zeros(len(pts))], axis=1)
             d = d - (d * mode).sum() / (mode * mode).sum() * mode
         e_u.append(np.abs(d).max() / np.abs(ru).max())
+    return np.array(e_cell), np.array(e_u)
+
+
+@pytest.mark.parametrize("case", ["steel304", "ta", "dp"])
+def test_reference_run_reproduces_stored_fields(case, tmp_path):
+    means, full, usol, pts, ref = _replay(case, tmp_path / "a")
+    t_mean, t_cell, t_u = TOL[case]
+    rm = ref["mean_sigma"]
+    # sigma_zz and von Mises carry the load; the lateral means are small, compare on the zz scale
+    e_mean = (np.abs(means - rm) / np.abs(rm[:, 2:3])).max(axis=1)
+    steps_list = ref["full_steps"]
+    e_cell, e_u = _field_errors(case, full, usol, pts, ref["full_sigma"], ref["full_sol"], steps_list)
     fmt = lambda v: " ".join(f"{x:.2e}" for x in v)
     print(f"{case}: mean err max {e_mean.max():.2e} (step {e_mean.argmax() + 1}), cell {fmt(e_cell)}, u {fmt(e_u)}")
     assert e_mean.max() < t_mean, (case, e_mean)
-    assert max(e_cell) < t_cell, (case, e_cell)
-    assert max(e_u) < t_u, (case, e_u)
+    if case == "steel304":
+        # the bar per stored step is the spread of two of this side's own runs whose inputs
+        # differ by one rounding (the driven displacement scaled by 1 + 2^-50): the reference's
+        # stored run is one more such sample
+        _, full2, usol2, _, _ = _replay(case, tmp_path / "b", disp_scale=1.0 + 2.0 ** -50)
+        own_s = np.stack([full[int(i)] for i in steps_list])
+        own_u = np.stack([usol[int(i)] for i in steps_list])
+        s_cell, s_u = _field_errors(case, full2, usol2, pts, own_s, own_u, steps_list)
+        print(f"{case}: own spread cell {fmt(s_cell)}, u {fmt(s_u)}")
+        assert (e_cell <= np.maximum(t_cell, 10.0 * s_cell)).all(), (case, e_cell, s_cell)
+        assert (e_u <= np.maximum(t_u, 10.0 * s_u)).all(), (case, e_u, s_u)
+    else:
+        assert e_cell.max() < t_cell, (case, e_cell)
+        assert e_u.max() < t_u, (case, e_u)
