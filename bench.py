@@ -351,6 +351,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mixed-steps", type=int, default=2, help="timed load steps in the other precision (0: none)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip both adjoint gradients")
+    ap.add_argument("--no-history-adjoint", action="store_true", help="skip the calibration workload's history adjoint")
     ap.add_argument("--park", type=int, default=-1,
                     help="park the point kernel's lanes when at most this many of a wave still iterate "
                          "(0: never; -1: the library's default)")
@@ -532,9 +533,10 @@ def main():
                 a64["szz"] - szz[0]) / abs(szz[0]))
             detail["adjoint_load_step"] = a64
             log(f"[bench] load-step adjoint: {json.dumps(a64)}")
-        log("[bench] history adjoint (calibration workload) ...")
-        detail["adjoint_history"] = history_adjoint(n=args.history_adjoint_mesh)
-        log(f"[bench] history adjoint: {json.dumps(detail['adjoint_history'])}")
+        if not args.no_history_adjoint:
+            log("[bench] history adjoint (calibration workload) ...")
+            detail["adjoint_history"] = history_adjoint(n=args.history_adjoint_mesh)
+            log(f"[bench] history adjoint: {json.dumps(detail['adjoint_history'])}")
 
     cpu = None
     if not args.no_cpu:

@@ -25,10 +25,17 @@ static hipEvent_t prof_event(Prof* p) {
   return e;
 }
 
+// Launches of a class between two timed ones: the Krylov classes (tens of thousands of short
+// launches per load step) are sampled; every launch of the point and assembly classes (a few
+// hundred of milliseconds-long launches, several kernels sharing a class) is timed.
+static int prof_every(const Prof* p, int cls) {
+  return (cls == PC_SPMV || cls == PC_KVEC || cls == PC_KREPLACE) ? p->sample : 1;
+}
+
 int prof_begin(Prof* p, int cls, hipStream_t st) {
   if (!p || !p->on) return -1;
   const int64_t k = p->launches[cls]++;
-  if (k % p->sample) return -1;
+  if (k % prof_every(p, cls)) return -1;
   if (p->pend.size() >= 8192) prof_flush(p);
   Prof::Pending e{cls, prof_event(p), prof_event(p)};
   (void)hipEventRecord(e.a, st);
@@ -39,7 +46,7 @@ int prof_begin(Prof* p, int cls, hipStream_t st) {
 bool prof_sample(Prof* p, int cls, hipEvent_t* start, hipEvent_t* stop) {
   if (!p || !p->on) return false;
   const int64_t k = p->launches[cls]++;
-  if (k % p->sample) return false;
+  if (k % prof_every(p, cls)) return false;
   if (p->pend.size() >= 8192) prof_flush(p);
   Prof::Pending e{cls, prof_event(p), prof_event(p)};
   p->pend.push_back(e);

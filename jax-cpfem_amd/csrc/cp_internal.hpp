@@ -117,8 +117,8 @@ void launch_grad_field(const Geom& G, const double* lam, const uint8_t* is_bc, d
 void launch_transpose(const BellView& M, const int32_t* tslot, const double* A, double* At, hipStream_t st);
 
 // ---------------------------------------------------------------- profiling
-// Optional per-kernel-class timing with HIP events recorded on the launch stream around
-// every `sample`-th launch of a class; the rest are only counted.
+// Optional per-kernel-class timing with HIP events recorded on the launch stream around every
+// `sample`-th launch of the Krylov classes and every launch of the others; all are counted.
 enum ProfClass {
   PC_ELEMENT_QP = 0,  // constitutive update + tangent (k_element_qp)
   PC_ELEMENT_K,       // element stiffness / residual (k_element_K, k_element_R)

@@ -224,56 +224,74 @@ __global__ void __launch_bounds__(QBS) k_element_qp_park(Geom G, const double* _
 
 // ---------------------------------------------------------------------------------
 // Element stiffness / residual: one thread per (cell, local node a); writes the 8 node-pair
-// blocks of row-block a (KE[cell][a][b][i][k]) and r_e[cell][a][i].
-__global__ void __launch_bounds__(256) k_element_K(Geom G, const double* __restrict__ wP, const double* __restrict__ wA,
-                                                  double* __restrict__ KE, double* __restrict__ RE) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= G.n_qp) return;
+// blocks of row-block a (KE[cell][a][b][i][k]) and r_e[cell][a][i].  A block takes 16 cells
+// (128 points, contiguous in the [component][Q] layouts) and stages their JxW P and, one
+// row index i at a time, the 27 tangent components dP_(ij)/dH_(kl) of all 128 points in LDS
+// with coalesced loads: each value is read from HBM once, where eight threads of a cell
+// reading it from global memory re-fetched the tangent ~7 times (L2 misses) per launch.
+constexpr int EK_CELLS = 16;
+constexpr int EK_BS = 8 * EK_CELLS;  // threads = points per block
+
+__global__ void __launch_bounds__(EK_BS) k_element_K(Geom G, const double* __restrict__ wP, const double* __restrict__ wA,
+                                                    double* __restrict__ KE, double* __restrict__ RE) {
+  __shared__ double sP[9][EK_BS];
+  __shared__ double sA[27][EK_BS];
+  const int64_t q0 = (int64_t)blockIdx.x * EK_BS;  // first point of the block
+  const int tid = threadIdx.x;
+  const int64_t t = q0 + tid;                      // (cell, a) of this thread
+  const bool valid = t < G.n_qp;
   const int64_t cell = t >> 3;
-  const int a = (int)(t & 7);
+  const int a = tid & 7;
+  const int cl = tid >> 3;  // cell within the block
+#pragma unroll
+  for (int c = 0; c < 9; ++c) sP[c][tid] = valid ? wP[c * G.n_qp + t] : 0.0;
   double K[72];
+#pragma unroll
+  for (int k = 0; k < 72; ++k) K[k] = 0.0;
   double re[3] = {0.0, 0.0, 0.0};
+#pragma unroll  // i indexes the register arrays K and re: unrolled, no scratch
+  for (int i = 0; i < 3; ++i) {
+    __syncthreads();  // sA free (previous i), sP written
 #pragma unroll
-  for (int i = 0; i < 72; ++i) K[i] = 0.0;
+    for (int c = 0; c < 27; ++c) sA[c][tid] = valid ? wA[(int64_t)(27 * i + c) * G.n_qp + t] : 0.0;
+    __syncthreads();
+    if (valid) {
 #pragma unroll 1
-  for (int q = 0; q < 8; ++q) {
-    const int64_t qp = cell * 8 + q;
-    double ga[3], gb[24];
+      for (int q = 0; q < 8; ++q) {
+        const int p = 8 * cl + q;
+        const int64_t qp = cell * 8 + q;
+        double ga[3], gb[24];
 #pragma unroll
-    for (int c = 0; c < 24; ++c) gb[c] = sg_at(G, c, qp);
-    ga[0] = sg_at(G, 3 * a, qp);
-    ga[1] = sg_at(G, 3 * a + 1, qp);
-    ga[2] = sg_at(G, 3 * a + 2, qp);
+        for (int c = 0; c < 24; ++c) gb[c] = sg_at(G, c, qp);
+        ga[0] = sg_at(G, 3 * a, qp);
+        ga[1] = sg_at(G, 3 * a + 1, qp);
+        ga[2] = sg_at(G, 3 * a + 2, qp);
+        double sr = 0.0;
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      double s = 0.0;
+        for (int j = 0; j < 3; ++j) sr += sP[3 * i + j][p] * ga[j];
+        re[i] += sr;
+        double Z[9];  // Z[k][l] = sum_j ga_j A[(i j),(k l)]
 #pragma unroll
-      for (int j = 0; j < 3; ++j) s += wP[(3 * i + j) * G.n_qp + qp] * ga[j];
-      re[i] += s;
-    }
-    const double* Aq = wA + qp;  // [81][Q]: component m at Aq[m * Q]
+        for (int kl = 0; kl < 9; ++kl) {
+          double z = 0.0;
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      double Z[9];  // Z[k][l] = sum_j ga_j A[(i j),(k l)]
+          for (int j = 0; j < 3; ++j) z += ga[j] * sA[9 * j + kl][p];
+          Z[kl] = z;
+        }
 #pragma unroll
-      for (int kl = 0; kl < 9; ++kl) {
-        double s = 0.0;
+        for (int b = 0; b < 8; ++b)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) s += ga[j] * Aq[(int64_t)(9 * (3 * i + j) + kl) * G.n_qp];
-        Z[kl] = s;
+          for (int k = 0; k < 3; ++k)
+            K[b * 9 + i * 3 + k] += Z[3 * k] * gb[3 * b] + Z[3 * k + 1] * gb[3 * b + 1] + Z[3 * k + 2] * gb[3 * b + 2];
       }
-#pragma unroll
-      for (int b = 0; b < 8; ++b)
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-          K[b * 9 + i * 3 + k] += Z[3 * k] * gb[3 * b] + Z[3 * k + 1] * gb[3 * b + 1] + Z[3 * k + 2] * gb[3 * b + 2];
     }
   }
+  if (!valid) return;
   double2* out = reinterpret_cast<double2*>(KE + t * 72);
 #pragma unroll
-  for (int i = 0; i < 36; ++i) out[i] = make_double2(K[2 * i], K[2 * i + 1]);
+  for (int k = 0; k < 36; ++k) out[k] = make_double2(K[2 * k], K[2 * k + 1]);
 #pragma unroll
-  for (int i = 0; i < 3; ++i) RE[t * 3 + i] = re[i];
+  for (int k = 0; k < 3; ++k) RE[t * 3 + k] = re[k];
 }
 
 // ---------------------------------------------------------------------------------
@@ -481,7 +499,7 @@ void launch_element_K(const Geom& G, const double* wP, const double* wA, double*
   const int bs = 256;
   const int64_t nb = (G.n_qp + bs - 1) / bs;
   if (wA)
-    hipLaunchKernelGGL(k_element_K, dim3((unsigned)nb), dim3(bs), 0, st, G, wP, wA, KE, RE);
+    hipLaunchKernelGGL(k_element_K, dim3((unsigned)((G.n_qp + EK_BS - 1) / EK_BS)), dim3(EK_BS), 0, st, G, wP, wA, KE, RE);
   else
     hipLaunchKernelGGL(k_element_R, dim3((unsigned)nb), dim3(bs), 0, st, G, wP, RE);
 }
