@@ -62,8 +62,8 @@ def spmv_bytes(prob, value_bytes=8):
 
 
 def point_flops(n_slip, tangent_points, newton_iters, residual_evals):
-    """Algorithmic FP64 flops of the assembly's point kernels (k_element_qp and the parked
-    points' k_element_qp_park) from measured totals: the reference algorithm's per-unit counts
+    """Algorithmic FP64 flops of the assembly's point kernels (k_element_res, the parked points'
+    k_element_qp_park and k_point_tangent) from measured totals: the reference algorithm's per-unit counts
     (FMA = 2 flops; div, abs, sign, pow, sqrt = 1 each), F_res(N) = 41 N + 350 per residual
     evaluation, F_jac(N) = 165 N + 2574 per local Jacobian and F_LU = 648 per 9x9 LU and solve
     (once per iteration), F_tan = 9300 per consistent tangent (assemblies with the Jacobian
@@ -438,8 +438,10 @@ def main():
     form = prob.ctx.krylov_config()[0]
     names = spmv_kernel_names(form, tm)
     tkey = f"c{args.config}_{n_mesh}"
+    point_ms = lambda: (kp["element_qp"]["est_total_ms"] + kp["element_qp_park"]["est_total_ms"]
+                        + kp["point_tangent"]["est_total_ms"])
     dom = max(("krylov_spmv", "element_qp", "element_K", "gather_matrix", "krylov_vector", "point_local"),
-              key=lambda c: kp[c]["est_total_ms"] + (kp["element_qp_park"]["est_total_ms"] if c == "element_qp" else 0))
+              key=lambda c: point_ms() if c == "element_qp" else kp[c]["est_total_ms"])
     # Launches that move the matrix: two per BiCGSTAB iteration plus the initial residual of each
     # solve (the launches queued past convergence return at once and move nothing, so the class
     # time, sampled mean x all launches, is divided by the working launches only)
@@ -458,16 +460,16 @@ def main():
                          timing="hipExtLaunchKernelGGL start/stop events (kernel-bound) on every "
                                 f"{args.profile_sample}th launch of the class, on its launch stream")
     point_roof = None
-    qp, qpk = kp["element_qp"], kp["element_qp_park"]
+    qp = kp["element_qp"]
     if qp["avg_ms"] and qp["launches"]:
         tangent_launches = assemblies  # one Jacobian assembly per Newton iteration plus the initial one
         flops = point_flops(prob.num_slip_sys, Q * tangent_launches, work["newton_iters"], work["residual_evals"])
-        t_ms = qp["est_total_ms"] + qpk["est_total_ms"]
+        t_ms = point_ms()
         ach_tf = flops / (t_ms * 1e-3) / 1e12
         point_roof = dict(bound="fp64 valu", achieved=round(ach_tf, 3), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
                           frac=round(ach_tf / FP64_PEAK_TFLOPS, 4),
-                          kernel=f"cpfem::k_element_qp<{prob.num_slip_sys}> + k_element_qp_park (local Newton + P + "
-                                 "tangent per point)",
+                          kernel=f"cpfem::k_element_res<{prob.num_slip_sys}> + k_point_tangent + k_element_qp_park "
+                                 "(local Newton + P; consistent tangent per point)",
                           launches=qp["launches"], tangent_launches=tangent_launches,
                           avg_launch_ms=round(t_ms / qp["launches"], 4), park_lanes=args.park,
                           parked_points_per_launch=round(work["parked_points"] / qp["launches"], 1),

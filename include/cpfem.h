@@ -254,10 +254,11 @@ int cpfem_transpose(cpfem_ctx* ctx, void* stream, const double* mat, double* mat
 
 /* Kernel-class timing with HIP events on the launch stream (of the Krylov classes 4, 5 and 8
    every `sample`-th launch is timed, of the others every launch; all are counted); enable
-   resets the counters.  Classes (index): 0 point update + tangent, 1 element
+   resets the counters.  Classes (index): 0 point update (local Newton, P), 1 element
    stiffness/residual, 2 matrix gather, 3 residual gather, 4 Krylov SpMV kernels, 5 Krylov
    vector kernels, 6 point stress/history, 7 copy, 8 residual replacement of the mixed form,
-   9 parked points of the point kernel.  No counterpart in the reference. */
+   9 parked points of the point kernel, 10 consistent tangent of the assembly.  No counterpart
+   in the reference. */
 int cpfem_profile(cpfem_ctx* ctx, int32_t enable, int32_t sample);
 int cpfem_profile_read(cpfem_ctx* ctx, int32_t n, double* ms, int64_t* launches, int64_t* timed);
 

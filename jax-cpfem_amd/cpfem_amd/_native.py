@@ -100,7 +100,7 @@ _SIGS = {
 }
 
 PROF_CLASSES = ("element_qp", "element_K", "gather_matrix", "gather_residual", "krylov_spmv", "krylov_vector",
-                "point_local", "copy", "krylov_replace", "element_qp_park")
+                "point_local", "copy", "krylov_replace", "element_qp_park", "point_tangent")
 
 EXPORTED = tuple(_SIGS)
 _lib = None

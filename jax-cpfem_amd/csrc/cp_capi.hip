@@ -124,6 +124,8 @@ struct cpfem_ctx {
   double* d_ubar = nullptr;
   // device: element work buffers
   double *d_wP = nullptr, *d_wA = nullptr, *d_KE = nullptr, *d_RE = nullptr, *d_sig = nullptr;
+  double* d_gi = nullptr;     // 1/g_old table of the Newton point kernel (allocated on first use)
+  double* d_sconv = nullptr;  // converged stresses [6][n_qp] for the tangent kernel (Jacobian assemblies)
   int32_t* d_status = nullptr;
   unsigned long long* d_counts = nullptr;  // [0] local Newton iterations, [1] residual evaluations, [2] parked points
   // parking of the assembly point kernel's slow lanes (allocated on first use)
@@ -398,7 +400,15 @@ static SolveCfg solve_cfg(const CpfemParams* p) {
 static int ensure_matrix_buffers(cpfem_ctx* c) {
   int rc;
   if (!c->d_wA && (rc = dalloc(&c->d_wA, (size_t)81 * c->n_qp))) return rc;
+  if (!c->d_sconv && (rc = dalloc(&c->d_sconv, (size_t)6 * c->n_qp))) return rc;
   if (!c->d_KE && (rc = dalloc(&c->d_KE, (size_t)576 * c->n_cells))) return rc;
+  return CPFEM_OK;
+}
+
+// 1/g_old table of the Newton point kernel ([n_slip][n_qp]), allocated on first use
+static int ensure_point_scratch(cpfem_ctx* c) {
+  int rc;
+  if (!c->d_gi && (rc = dalloc(&c->d_gi, (size_t)c->n_slip * c->n_qp))) return rc;
   return CPFEM_OK;
 }
 
@@ -699,6 +709,7 @@ void cpfem_destroy(cpfem_ctx* c) {
   dfree(c->d_col); dfree(c->d_nslot); dfree(c->d_dslot); dfree(c->d_tslot);
   dfree(c->d_cptr); dfree(c->d_clist); dfree(c->d_nptr); dfree(c->d_nlist);
   dfree(c->d_isbc); dfree(c->d_nobc); dfree(c->d_ubar);
+  dfree(c->d_gi); dfree(c->d_sconv);
   dfree(c->d_wP); dfree(c->d_wA); dfree(c->d_KE); dfree(c->d_RE); dfree(c->d_sig); dfree(c->d_status); dfree(c->d_counts);
   dfree(c->d_park_idx); dfree(c->d_park_it); dfree(c->d_park_s); dfree(c->d_park_ctr);
   dfree(c->d_partial); dfree(c->d_scalar); dfree(c->d_counter);
@@ -798,12 +809,14 @@ int cpfem_assemble(cpfem_ctx* c, void* stream, const double* dofs, const CpfemSt
   int rc;
   if ((rc = check_state(c, state, params))) return rc;
   if (mat && (rc = ensure_matrix_buffers(c))) return rc;
+  if ((rc = ensure_point_scratch(c))) return rc;
   hipStream_t st = (hipStream_t)stream;
   const Geom G = c->geom();
   const StateView sv = state_view(state, c->n_qp);
   const MatView mv = mat_view(params);
   const SolveCfg cfg = solve_cfg(params);
   double* wA = mat ? c->d_wA : nullptr;
+  double* sout = mat ? c->d_sconv : nullptr;  // converged stresses for the tangent kernel
   Prof* pf = &c->prof;
   PointPark pk{c->d_park_idx, c->d_park_s, c->d_park_it, c->d_park_ctr, c->n_qp, c->park_lanes};
   if (pk.lanes > 0) {
@@ -813,16 +826,24 @@ int cpfem_assemble(cpfem_ctx* c, void* stream, const double* dofs, const CpfemSt
   }
   int sl = prof_begin(pf, PC_ELEMENT_QP, st);
   if (c->n_slip == 12)
-    launch_element_qp<12>(G, dofs, sv, mv, c->tab12, cfg, c->d_wP, wA, iters, c->d_status, c->d_counts, pk, st);
+    launch_element_qp<12>(G, dofs, sv, mv, c->tab12, cfg, c->d_wP, sout, iters, c->d_status, c->d_counts, pk, c->d_gi, st);
   else
-    launch_element_qp<24>(G, dofs, sv, mv, c->tab24, cfg, c->d_wP, wA, iters, c->d_status, c->d_counts, pk, st);
+    launch_element_qp<24>(G, dofs, sv, mv, c->tab24, cfg, c->d_wP, sout, iters, c->d_status, c->d_counts, pk, c->d_gi, st);
   prof_end(pf, sl, st);
   if (pk.lanes > 0) {
     sl = prof_begin(pf, PC_ELEMENT_PARK, st);
     if (c->n_slip == 12)
-      launch_element_qp_park<12>(G, dofs, sv, mv, c->tab12, cfg, c->d_wP, wA, iters, c->d_status, c->d_counts, pk, st);
+      launch_element_qp_park<12>(G, dofs, sv, mv, c->tab12, cfg, c->d_wP, sout, iters, c->d_status, c->d_counts, pk, st);
     else
-      launch_element_qp_park<24>(G, dofs, sv, mv, c->tab24, cfg, c->d_wP, wA, iters, c->d_status, c->d_counts, pk, st);
+      launch_element_qp_park<24>(G, dofs, sv, mv, c->tab24, cfg, c->d_wP, sout, iters, c->d_status, c->d_counts, pk, st);
+    prof_end(pf, sl, st);
+  }
+  if (mat) {
+    sl = prof_begin(pf, PC_TANGENT, st);
+    if (c->n_slip == 12)
+      launch_point_tangent<12>(G, dofs, sv, mv, c->tab12, sout, wA, st);
+    else
+      launch_point_tangent<24>(G, dofs, sv, mv, c->tab24, sout, wA, st);
     prof_end(pf, sl, st);
   }
   sl = prof_begin(pf, PC_ELEMENT_K, st);

@@ -71,15 +71,22 @@ struct PointPark {
   int lanes;
 };
 
+// The assembly's point kernels: launch_element_qp runs the local Newton and writes JxW P and,
+// if sout ([6][n_qp]) is set, the converged stress, keeping the 1/g_old table in gi
+// ([N][n_qp] device scratch); launch_point_tangent forms the JxW-weighted consistent tangent
+// wA ([81][n_qp]) from that stress.
 template <int N>
 void launch_element_qp(const Geom& G, const double* u, const StateView& sv, const MatView& mv, const SlipTable<N>& tab,
-                       const SolveCfg& cfg, double* wP, double* wA, int32_t* iters, int32_t* status,
-                       unsigned long long* counts, const PointPark& park, hipStream_t st);
+                       const SolveCfg& cfg, double* wP, double* sout, int32_t* iters, int32_t* status,
+                       unsigned long long* counts, const PointPark& park, double* gi, hipStream_t st);
 // The second kernel over the parked points (grid sized to the device, chunks taken atomically).
 template <int N>
 void launch_element_qp_park(const Geom& G, const double* u, const StateView& sv, const MatView& mv, const SlipTable<N>& tab,
-                            const SolveCfg& cfg, double* wP, double* wA, int32_t* iters, int32_t* status,
+                            const SolveCfg& cfg, double* wP, double* sout, int32_t* iters, int32_t* status,
                             unsigned long long* counts, const PointPark& park, hipStream_t st);
+template <int N>
+void launch_point_tangent(const Geom& G, const double* u, const StateView& sv, const MatView& mv, const SlipTable<N>& tab,
+                          const double* sin, double* wA, hipStream_t st);
 template <int N>
 void launch_local(int64_t n_qp, const double* Hd, const Geom& G, const double* u, const StateView& sv, const MatView& mv,
                   const SlipTable<N>& tab, const SolveCfg& cfg, int self_block, double* P, double* A, double* sig,
@@ -120,7 +127,7 @@ void launch_transpose(const BellView& M, const int32_t* tslot, const double* A, 
 // Optional per-kernel-class timing with HIP events recorded on the launch stream around every
 // `sample`-th launch of the Krylov classes and every launch of the others; all are counted.
 enum ProfClass {
-  PC_ELEMENT_QP = 0,  // constitutive update + tangent (k_element_qp)
+  PC_ELEMENT_QP = 0,  // constitutive update (local Newton, P) of the assembly (k_element_res)
   PC_ELEMENT_K,       // element stiffness / residual (k_element_K, k_element_R)
   PC_GATHER_MAT,      // block-ELL gather (k_gather_matrix)
   PC_GATHER_RES,      // residual gather (k_gather_residual)
@@ -130,6 +137,7 @@ enum ProfClass {
   PC_COPY,            // plain copy (bandwidth calibration)
   PC_KREPLACE,        // residual replacement of the mixed form (k_kreplace)
   PC_ELEMENT_PARK,    // parked points of the assembly's point kernel (k_element_qp_park)
+  PC_TANGENT,         // consistent tangent from the converged stresses (k_point_tangent)
   PC_COUNT
 };
 

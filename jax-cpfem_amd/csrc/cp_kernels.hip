@@ -68,30 +68,41 @@ __device__ __forceinline__ uint32_t wave_count(uint32_t v) {
 
 // Point q's inputs: F = I + u_grad from the dofs, Fp_inv_old and R, and the solver set-up
 // (rotated stiffness, 1/g_old in LDS).
+// With gi (a [N][n_qp] device array) the 1/g_old table lives there instead of in LDS, and
+// sh holds only the d dgamma / d tau table.
 template <int N>
 __device__ __forceinline__ void start_point(const Geom& G, const double* __restrict__ u, const StateView& sv,
                                             const MatView& mv, int64_t q, double* sh, Point<N>& pt, double* F,
-                                            double* Fpi) {
+                                            double* Fpi, double* gi = nullptr) {
   double H[9], R[9];
   ugrad_from_dofs(G, u, q, H);
 #pragma unroll
   for (int i = 0; i < 9; ++i) F[i] = H[i] + ((i % 4) == 0 ? 1.0 : 0.0);
   load_point(sv, q, Fpi, R);
   const PointMat m = load_mat(mv, q);
-  pt.setup(F, Fpi, R, sv.g + q, sv.ld, m, sh + threadIdx.x, sh + N * QBS + threadIdx.x, QBS);
+  if (gi)
+    pt.setup(F, Fpi, R, sv.g + q, sv.ld, m, sh + threadIdx.x, gi + q, QBS, G.n_qp);
+  else
+    pt.setup(F, Fpi, R, sv.g + q, sv.ld, m, sh + threadIdx.x, sh + N * QBS + threadIdx.x, QBS);
 }
 
-// Outputs of a converged point: JxW P, JxW dP/du_grad, iterations, status bits.
+// Outputs of a converged point: JxW P, JxW dP/du_grad (wA), the converged stress (sout,
+// [6][Q]), iterations, status bits.
 template <int N>
 __device__ __forceinline__ void finish_point(const Geom& G, const SlipTable<N>& tab, int64_t q, Point<N>& pt,
                                              const double* F, const double* Fpi, const double* s, int it, int32_t st,
                                              double* __restrict__ wP, double* __restrict__ wA,
-                                             int32_t* __restrict__ iters, int32_t* __restrict__ status) {
+                                             int32_t* __restrict__ iters, int32_t* __restrict__ status,
+                                             double* __restrict__ sout = nullptr) {
   const double w = jxw_at(G, q);
   double P[9];
   pk1<N>(pt, F, Fpi, s, P);
 #pragma unroll
   for (int i = 0; i < 9; ++i) wP[i * G.n_qp + q] = w * P[i];
+  if (sout) {
+#pragma unroll
+    for (int j = 0; j < 6; ++j) sout[j * G.n_qp + q] = s[j];
+  }
   if (wA) {
     tangent<N>(pt, tab, F, Fpi, s, [&](int m9, const double* dP) {
 #pragma unroll
@@ -115,9 +126,16 @@ __device__ __forceinline__ void add_counts(unsigned long long* counts, uint32_t 
 constexpr double DBL_MAXV = 1.7976931348623157e308;
 
 // ---------------------------------------------------------------------------------
-// Quadrature-point kernel of the assembly: H from dofs, local Newton, JxW-weighted P (SoA
-// [9][Q]) and, if wA, the JxW-weighted consistent tangent, also structure-of-arrays
-// ([81][Q]: each of the 81 stores of a wave writes 512 contiguous bytes).
+// Quadrature-point kernels of the assembly.  k_element_res: H from dofs, local Newton,
+// JxW-weighted P (SoA [9][Q]) and, for Jacobian assemblies, the converged stress (sout,
+// [6][Q]); k_point_tangent then forms the JxW-weighted consistent tangent ([81][Q]: each of
+// the 81 stores of a wave writes 512 contiguous bytes) from the stored stress.  Without the
+// tangent the Newton kernel fits 256 VGPRs (a few spilled) and keeps the 1/g_old table in a
+// device array instead of LDS, so two waves share a SIMD; the tangent kernel needs ~440 VGPRs
+// (one wave per SIMD) but runs one residual evaluation and the tangent per point, no Newton
+// loop whose lanes wait on each other.  The tangent kernel re-evaluates the residual at the
+// stored stress, which gives bitwise the state the Newton kernel ended with (same inline
+// code, multiply-adds fused per source expression).
 // The lanes of a wave run their Newton iterations in lockstep; a wave whose points need very
 // different iteration counts would otherwise run until its slowest lane converges.  With
 // parking (pk.lanes > 0), once at most pk.lanes lanes of the wave are still iterating, those
@@ -127,12 +145,12 @@ constexpr double DBL_MAXV = 1.7976931348623157e308;
 // re-evaluated there.  counts (optional): [0] += local Newton iterations, [1] += residual
 // evaluations (one atomic per wave), the measured inputs of the algorithmic flop count.
 template <int N>
-__global__ void __launch_bounds__(QBS) k_element_qp(Geom G, const double* __restrict__ u, StateView sv, MatView mv,
-                                                   SlipTable<N> tab, SolveCfg cfg, double* __restrict__ wP,
-                                                   double* __restrict__ wA, int32_t* __restrict__ iters,
-                                                   int32_t* __restrict__ status, unsigned long long* __restrict__ counts,
-                                                   PointPark pk) {
-  __shared__ double sh[2 * N * QBS];
+__device__ __forceinline__ void element_qp_body(const Geom& G, const double* __restrict__ u, const StateView& sv,
+                                                const MatView& mv, const SlipTable<N>& tab, const SolveCfg& cfg,
+                                                double* __restrict__ wP, double* __restrict__ sout,
+                                                int32_t* __restrict__ iters, int32_t* __restrict__ status,
+                                                unsigned long long* __restrict__ counts, const PointPark& pk, double* sh,
+                                                double* gi) {
   const int64_t q = (int64_t)blockIdx.x * QBS + threadIdx.x;
   const bool valid = q < G.n_qp;
   Point<N> pt;
@@ -144,7 +162,7 @@ __global__ void __launch_bounds__(QBS) k_element_qp(Geom G, const double* __rest
   pt.nev = 0;
   pt.nr = 0.0;
   if (valid) {
-    start_point<N>(G, u, sv, mv, q, sh, pt, F, Fpi);
+    start_point<N>(G, u, sv, mv, q, sh, pt, F, Fpi, gi);
     pt.eval(tab, s);
   }
   bool act = valid && pt.nr > cfg.tol;
@@ -180,10 +198,41 @@ __global__ void __launch_bounds__(QBS) k_element_qp(Geom G, const double* __rest
   uint32_t n_it = 0;
   if (valid && !act) {
     if (!(fabs(pt.nr) <= DBL_MAXV)) st |= PT_NONFINITE;
-    finish_point<N>(G, tab, q, pt, F, Fpi, s, it, st, wP, wA, iters, status);
+    finish_point<N>(G, tab, q, pt, F, Fpi, s, it, st, wP, nullptr, iters, status, sout);
     n_it = (uint32_t)it;
   }
   add_counts(counts, n_it, valid ? (uint32_t)pt.nev : 0u);
+}
+
+template <int N>
+__global__ void __launch_bounds__(QBS) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_element_res(Geom G, const double* __restrict__ u, StateView sv, MatView mv, SlipTable<N> tab, SolveCfg cfg,
+              double* __restrict__ wP, double* __restrict__ sout, int32_t* __restrict__ iters,
+              int32_t* __restrict__ status, unsigned long long* __restrict__ counts, PointPark pk,
+              double* __restrict__ gi) {
+  __shared__ double sh[N * QBS];
+  element_qp_body<N>(G, u, sv, mv, tab, cfg, wP, sout, iters, status, counts, pk, sh, gi);
+}
+
+// Consistent tangent of every point from its converged stress sin ([6][Q]).
+template <int N>
+__global__ void __launch_bounds__(QBS) k_point_tangent(Geom G, const double* __restrict__ u, StateView sv, MatView mv,
+                                                      SlipTable<N> tab, const double* __restrict__ sin,
+                                                      double* __restrict__ wA) {
+  __shared__ double sh[2 * N * QBS];
+  const int64_t q = (int64_t)blockIdx.x * QBS + threadIdx.x;
+  if (q >= G.n_qp) return;
+  Point<N> pt;
+  double F[9], Fpi[9], s[6];
+  start_point<N>(G, u, sv, mv, q, sh, pt, F, Fpi);
+#pragma unroll
+  for (int j = 0; j < 6; ++j) s[j] = sin[j * G.n_qp + q];
+  pt.eval(tab, s);
+  const double w = jxw_at(G, q);
+  tangent<N>(pt, tab, F, Fpi, s, [&](int m9, const double* dP) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) wA[(int64_t)(9 * i + m9) * G.n_qp + q] = w * dP[i];
+  });
 }
 
 // The parked points of k_element_qp, one queue entry per lane: each point's stress iterate is
@@ -195,7 +244,7 @@ __global__ void __launch_bounds__(QBS) k_element_qp(Geom G, const double* __rest
 template <int N>
 __global__ void __launch_bounds__(QBS) k_element_qp_park(Geom G, const double* __restrict__ u, StateView sv, MatView mv,
                                                         SlipTable<N> tab, SolveCfg cfg, double* __restrict__ wP,
-                                                        double* __restrict__ wA, int32_t* __restrict__ iters,
+                                                        double* __restrict__ sout, int32_t* __restrict__ iters,
                                                         int32_t* __restrict__ status,
                                                         unsigned long long* __restrict__ counts, PointPark pk) {
   __shared__ double sh[2 * N * QBS];
@@ -215,7 +264,7 @@ __global__ void __launch_bounds__(QBS) k_element_qp_park(Geom G, const double* _
     int it = pk.it[k];
     int32_t st = 0;
     pt.iterate(tab, cfg.tol, cfg.max_sub_step, cfg.max_iter, s, &it, &st);
-    finish_point<N>(G, tab, q, pt, F, Fpi, s, it, st, wP, wA, iters, status);
+    finish_point<N>(G, tab, q, pt, F, Fpi, s, it, st, wP, nullptr, iters, status, sout);
     n_it = (uint32_t)it;
     n_ev = (uint32_t)pt.nev;
   }
@@ -466,20 +515,27 @@ __global__ void k_bc_x0(int64_t n, const uint8_t* __restrict__ is_bc, const doub
 // Launch helpers (host)
 template <int N>
 void launch_element_qp(const Geom& G, const double* u, const StateView& sv, const MatView& mv, const SlipTable<N>& tab,
-                       const SolveCfg& cfg, double* wP, double* wA, int32_t* iters, int32_t* status,
-                       unsigned long long* counts, const PointPark& park, hipStream_t st) {
+                       const SolveCfg& cfg, double* wP, double* sout, int32_t* iters, int32_t* status,
+                       unsigned long long* counts, const PointPark& park, double* gi, hipStream_t st) {
   const int64_t nb = (G.n_qp + QBS - 1) / QBS;
-  hipLaunchKernelGGL(k_element_qp<N>, dim3((unsigned)nb), dim3(QBS), 0, st, G, u, sv, mv, tab, cfg, wP, wA, iters, status,
-                     counts, park);
+  hipLaunchKernelGGL(k_element_res<N>, dim3((unsigned)nb), dim3(QBS), 0, st, G, u, sv, mv, tab, cfg, wP, sout, iters, status,
+                     counts, park, gi);
 }
 
 template <int N>
 void launch_element_qp_park(const Geom& G, const double* u, const StateView& sv, const MatView& mv, const SlipTable<N>& tab,
-                            const SolveCfg& cfg, double* wP, double* wA, int32_t* iters, int32_t* status,
+                            const SolveCfg& cfg, double* wP, double* sout, int32_t* iters, int32_t* status,
                             unsigned long long* counts, const PointPark& park, hipStream_t st) {
   const int64_t nb = (park.cap + QBS - 1) / QBS;
-  hipLaunchKernelGGL(k_element_qp_park<N>, dim3((unsigned)nb), dim3(QBS), 0, st, G, u, sv, mv, tab, cfg, wP, wA, iters,
+  hipLaunchKernelGGL(k_element_qp_park<N>, dim3((unsigned)nb), dim3(QBS), 0, st, G, u, sv, mv, tab, cfg, wP, sout, iters,
                      status, counts, park);
+}
+
+template <int N>
+void launch_point_tangent(const Geom& G, const double* u, const StateView& sv, const MatView& mv, const SlipTable<N>& tab,
+                          const double* sin, double* wA, hipStream_t st) {
+  const int64_t nb = (G.n_qp + QBS - 1) / QBS;
+  hipLaunchKernelGGL(k_point_tangent<N>, dim3((unsigned)nb), dim3(QBS), 0, st, G, u, sv, mv, tab, sin, wA);
 }
 
 template <int N>
@@ -541,10 +597,12 @@ void launch_cell_average(const Geom& G, const double* sig, double* out, hipStrea
 #define CPFEM_INST(N)                                                                                                      \
   template void launch_element_qp<N>(const Geom&, const double*, const StateView&, const MatView&, const SlipTable<N>&,    \
                                      const SolveCfg&, double*, double*, int32_t*, int32_t*, unsigned long long*,           \
-                                     const PointPark&, hipStream_t);                                                       \
+                                     const PointPark&, double*, hipStream_t);                                              \
   template void launch_element_qp_park<N>(const Geom&, const double*, const StateView&, const MatView&,                    \
                                           const SlipTable<N>&, const SolveCfg&, double*, double*, int32_t*, int32_t*,      \
                                           unsigned long long*, const PointPark&, hipStream_t);                             \
+  template void launch_point_tangent<N>(const Geom&, const double*, const StateView&, const MatView&, const SlipTable<N>&, \
+                                        const double*, double*, hipStream_t);                                              \
   template void launch_local<N>(int64_t, const double*, const Geom&, const double*, const StateView&, const MatView&,      \
                                 const SlipTable<N>&, const SolveCfg&, int, double*, double*, double*, const StateOut&,     \
                                 int32_t*, int32_t*, hipStream_t);

@@ -210,8 +210,9 @@ struct Point {
   double C6[21];  // rotated stiffness, packed symmetric Voigt
   double p, c;    // 1/m, gamma0 dt
   double* gpv;    // d dgamma_a / d tau_a   [a * st]
-  double* giv;    // 1 / g_old_a            [a * st]
+  double* giv;    // 1 / g_old_a            [a * sti]
   int st;
+  int64_t sti;
   // results of the last residual evaluation
   double Lpc[9];  // sum_a dgamma_a d_a (x) n_a   (crystal frame)
   double Fe[9];
@@ -220,13 +221,15 @@ struct Point {
   int nev;        // residual evaluations since setup (work counter)
 
   CPFEM_HD double& gp(int a) { return gpv[a * st]; }
-  CPFEM_HD double& gi(int a) { return giv[a * st]; }
+  CPFEM_HD double& gi(int a) { return giv[a * sti]; }
 
+  // gp_store / gi_store: the per-slip arrays, at stride `stride` and `stride_gi` (-1: the same)
   CPFEM_HD void setup(const double* F, const double* Fpi_old, const double* Rin, const double* g_old, int64_t gstride,
-                      const PointMat& m, double* gp_store, double* gi_store, int stride) {
+                      const PointMat& m, double* gp_store, double* gi_store, int stride, int64_t stride_gi = -1) {
     gpv = gp_store;
     giv = gi_store;
     st = stride;
+    sti = stride_gi < 0 ? stride : stride_gi;
 #pragma unroll
     for (int i = 0; i < 9; ++i) R[i] = Rin[i];
     mm3(F, Fpi_old, A);
