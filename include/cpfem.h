@@ -198,8 +198,8 @@ int cpfem_norm(cpfem_ctx* ctx, void* stream, const double* v, double* out, int32
    out[2] = vector-kernel blocks. */
 int cpfem_krylov_config(const cpfem_ctx* ctx, int32_t* out);
 /* Forces the SpMV form of this context's BiCGSTAB (-1: picked by system size, the default).
-   For tests that run two forms on one system; form 0 is refused (CPFEM_EBADARG) on systems
-   with more than three 64-row tiles per CU.  No counterpart in the reference. */
+   For tests and measurements that run two forms on one system; form 0 is refused
+   (CPFEM_EBADARG) on systems of 2^28 rows or more.  No counterpart in the reference. */
 int cpfem_set_spmv_form(cpfem_ctx* ctx, int32_t form);
 
 /* Work counters of the assembly's point kernel, summed over its launches since the last

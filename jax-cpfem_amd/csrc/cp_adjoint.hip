@@ -26,7 +26,9 @@ __device__ __forceinline__ double vjp_w(const Geom& G, int64_t qp) { return G.un
 template <int N>
 __global__ void __launch_bounds__(VBS) k_local_vjp(Geom G, const double* __restrict__ u, StateView sv, MatView mv,
                                                   SlipTable<N> tab, SolveCfg cfg, int self_block, VjpArgs va) {
-  __shared__ double sh[(2 * N + 42) * VBS];
+  // The re-solve's per-slip-system columns (2N) and, once it has finished, the VJP's 42-double
+  // Jacobian column share one LDS allocation: each lane only touches its own column.
+  __shared__ double sh[(2 * N > 42 ? 2 * N : 42) * VBS];
   const int64_t q = (int64_t)blockIdx.x * VBS + threadIdx.x;
   if (q >= G.n_qp) return;
   const int64_t Q = G.n_qp;
@@ -95,7 +97,7 @@ __global__ void __launch_bounds__(VBS) k_local_vjp(Geom G, const double* __restr
   if (!va.gbar_old) mask &= ~4;
   if (!va.pbar) mask &= ~8;
   if (!va.Rbar) mask &= ~16;
-  point_vjp<N>(xin, s, tab, mv.ao, mv.dt, self_block, Pb, sb, Fb, gbq, slq, Q, sh + 2 * N * VBS + threadIdx.x, VBS, mask,
+  point_vjp<N>(xin, s, tab, mv.ao, mv.dt, self_block, Pb, sb, Fb, gbq, slq, Q, sh + threadIdx.x, VBS, mask,
                [&](int grp, int off, double xb) {
                  double* out = grp == 1 ? va.Hbar : grp == 2 ? va.Fpibar : grp == 4 ? va.gbar_old : grp == 8 ? va.pbar : va.Rbar;
                  double* dst = out + (int64_t)off * Q + q;

@@ -873,22 +873,24 @@ int krylov_spmv_form(int64_t n) {
 
 // SpMV blocks: a grid that fits the device in one round, each block walking a run of tiles
 // (at the serial forms' register use a CU holds 16 waves: two 512-thread or four 256-thread
-// blocks; the all-slots form holds three 256-thread blocks and is only picked when every
-// 64-row tile gets a block of its own; a grid beyond that runs in two rounds: on the 32^3
+// blocks; the all-slots form holds three 256-thread blocks and is only picked by size when
+// every 64-row tile gets a block of its own; a grid beyond that runs in two rounds: on the 32^3
 // matrix with 1024-thread blocks, 256 blocks took 46.3 us per BiCGSTAB iteration, 384 blocks
 // 61.7 us, 422 blocks 50.4 us).
 int krylov_spmv_blocks(int64_t n, int form) {
-  if (form == 2)
-    return (int)std::max<int64_t>(1, std::min<int64_t>((n + SpG<2>::BLOCK_ROWS - 1) / SpG<2>::BLOCK_ROWS, 2 * device_cus()));
-  return (int)std::max<int64_t>(1, std::min<int64_t>((n + SpG<1>::BLOCK_ROWS - 1) / SpG<1>::BLOCK_ROWS, 4 * device_cus()));
+  const int64_t per_cu = form == 2 ? 2 : form == 0 ? 3 : 4;
+  const int64_t rows = form == 2 ? SpG<2>::BLOCK_ROWS : SpG<1>::BLOCK_ROWS;
+  return (int)std::max<int64_t>(1, std::min<int64_t>((n + rows - 1) / rows, per_cu * device_cus()));
 }
 int krylov_spmv_blocks(int64_t n) { return krylov_spmv_blocks(n, krylov_spmv_form(n)); }
 
-// Whether a form may run on n rows: the all-slots form only where every 64-row tile gets a
-// block of its own at three blocks per CU (its lane offsets are 32-bit on that assumption).
+// Whether a form may run on n rows: the all-slots form keeps row and column byte offsets in
+// 32 bits, which holds below 2^28 rows.  (Forced on the 64^3 system, 823,875 rows, where its
+// grid of three blocks per CU walks runs of 64-row tiles, it takes 153.5 us per SpMV against
+// 148 us for the 128-row form.)
 bool krylov_form_allowed(int64_t n, int form) {
   if (form == 1 || form == 2) return true;
-  return form == 0 && (n + SpG<1>::BLOCK_ROWS - 1) / SpG<1>::BLOCK_ROWS <= 3 * device_cus();
+  return form == 0 && n < ((int64_t)1 << 28);
 }
 
 // Partial-sum stride that holds every form's grid (the form can be set per context).

@@ -76,7 +76,9 @@ def test_all_slots_form_reproduces_one_slot_form_bitwise():
     """At 16^3 the all-slots form (0) and the one-slot-at-a-time form (1) run the same grid of
     64-row tiles with the same products in the same order: forced on the same system, they give
     bitwise-equal solutions and iteration counts (FP64 and mixed); the 128-row form (2) solves
-    the same system to the same bar."""
+    the same system to the same bar.  Forced on a larger system (28^3), form 0's blocks walk
+    runs of tiles and meet the bar too."""
+    import cpfem_amd as C
     from cpfem_amd import _native as N
     from cpfem_amd.solver import krylov_solve
     prob, mat, b, x0 = _first_system(16)
@@ -94,6 +96,16 @@ def test_all_slots_form_reproduces_one_slot_form_bitwise():
         assert out[0, prec][1] == out[1, prec][1]
         x2, x1 = out[2, prec][0], out[1, prec][0]
         assert np.linalg.norm(x2 - x1) <= 1e-6 * np.linalg.norm(x1)
-    # form 0 is refused where tiles exceed three per CU
-    big, *_ = _first_system(28)
-    assert big.ctx._lib.cpfem_set_spmv_form(big.ctx.h, 0) == N.EBADARG
+    # forced where the 64-row tiles exceed three per CU (28^3: 1,144 tiles), the all-slots
+    # form's blocks walk runs of tiles; the solution meets the same true-residual bar as the
+    # size-picked form and lands within 1e-6 of it
+    big, bmat, bb, bx0 = _first_system(28)
+    xa = krylov_solve(big, bmat, bb, bx0).cpu().numpy()
+    N.check(big.ctx._lib.cpfem_set_spmv_form(big.ctx.h, 0))
+    assert _form(big) == 0
+    x0f = krylov_solve(big, bmat, bb, bx0).cpu().numpy()
+    A = C.BlockEllMatrix(big.ctx, bmat).to_scipy()
+    bh, x0h = bb.cpu().numpy(), bx0.cpu().numpy()
+    bar = 1e-10 * np.linalg.norm(bh) + 1e-9 * np.linalg.norm(bh - A @ x0h)
+    assert np.linalg.norm(A @ x0f - bh) <= bar
+    assert np.linalg.norm(x0f - xa) <= 1e-6 * np.linalg.norm(xa)

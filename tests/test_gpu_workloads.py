@@ -1,4 +1,6 @@
-"""End-to-end parity of the benchmark workloads (small instances) against the oracle.
+"""End-to-end parity of the benchmark workloads (small instances, and load step 1 of the
+dual-phase workload at 24^3: 110,592 points, 21 Newton iterations with line search) against
+the oracle.
 
 Each case builds the workload with cpfem_amd.configs (same seeds as the fixture generator,
 tests/golden/make_oracle_fixtures.py), runs the reference driver loop through the drop-in
@@ -18,7 +20,7 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 CASES = [("wl_c1_cu_8", 1, {"steps": 40}, 10), ("wl_c2_fcc_4", 2, {"n": 4}, 6), ("wl_c3_bcc24_4", 3, {"n": 4}, 4),
-         ("wl_c4_calib_4", 4, {"n": 4}, 4), ("wl_c5_dual_4", 5, {"n": 4}, 3)]
+         ("wl_c4_calib_4", 4, {"n": 4}, 4), ("wl_c5_dual_4", 5, {"n": 4}, 3), ("wl_c5_dual_24", 5, {"n": 24}, 1)]
 
 
 def _run(cfg, kw, steps, precision):
