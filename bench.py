@@ -76,9 +76,10 @@ def point_flops(n_slip, tangent_points, newton_iters, residual_evals):
 
 def spmv_kernel_names(form, tm):
     """Names of the BiCGSTAB SpMV kernel pair of an SpMV form (as the library reports it,
-    cpfem_krylov_config): 2 = 128-row tiles, 1 = 64-row tiles, 0 = 64-row tiles with every
-    slot's loads in flight."""
-    return {2: [f"cpfem::k_kpq<{tm}, 2>", f"cpfem::k_kst<{tm}, 2>"],
+    cpfem_krylov_config): 3 = one row per thread, 2 = 128-row tiles, 1 = 64-row tiles, 0 =
+    64-row tiles with every slot's loads in flight."""
+    return {3: [f"cpfem::k_kpq_flat<{tm}>", f"cpfem::k_kst_flat<{tm}>"],
+            2: [f"cpfem::k_kpq<{tm}, 2>", f"cpfem::k_kst<{tm}, 2>"],
             1: [f"cpfem::k_kpq<{tm}, 1>", f"cpfem::k_kst<{tm}, 1>"],
             0: [f"cpfem::k_kpq_all<{tm}>", f"cpfem::k_kst_all<{tm}>"]}[form]
 

@@ -158,7 +158,7 @@ struct cpfem_ctx {
   int spmv_form = -1;  // BiCGSTAB SpMV form: -1 = picked by size (krylov_spmv_form), else forced
   double* scratch = nullptr;  // device address of the fixed workspace array on this context's device
 
-  int krylov_form() const { return spmv_form >= 0 ? spmv_form : krylov_spmv_form(n_dofs); }
+  int krylov_form() const { return spmv_form >= 0 ? spmv_form : krylov_spmv_form(n_dofs, n_slots); }
 
   Geom geom() const {
     Geom G;
@@ -998,9 +998,10 @@ int cpfem_krylov_config(const cpfem_ctx* c, int32_t* out) {
 
 int cpfem_set_spmv_form(cpfem_ctx* c, int32_t form) {
   if (!c) return fail(CPFEM_EBADARG, "null context");
-  if (form < -1 || form > 2) return fail(CPFEM_EBADARG, "SpMV form must be -1 (by size), 0, 1 or 2");
-  if (form >= 0 && !krylov_form_allowed(c->n_dofs, form))
-    return fail(CPFEM_EBADARG, "the all-slots SpMV form needs at most three 64-row tiles per CU");
+  if (form < -1 || form > 3) return fail(CPFEM_EBADARG, "SpMV form must be -1 (by size), 0, 1, 2 or 3");
+  if (form >= 0 && !krylov_form_allowed(c->n_dofs, form, c->n_slots))
+    return fail(CPFEM_EBADARG, form == 3 ? "the one-row-per-thread SpMV form needs a 27-slot (hex8 box-like) mesh"
+                                         : "the all-slots SpMV form needs fewer than 2^28 rows");
   c->spmv_form = form;
   return CPFEM_OK;
 }

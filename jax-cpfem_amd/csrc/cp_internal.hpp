@@ -183,8 +183,8 @@ struct KrylovWork {
   int64_t n;
   int nb;            // blocks of the vector kernels (one row per thread, grid-stride)
   int nbs;           // blocks of the SpMV kernels (one or two 64-row groups x 4 slot parts per block)
-  int sf;            // SpMV form (krylov_spmv_form): 2 = 128-row tiles, 1 = 64-row tiles, 0 = 64-row
-                     // tiles with every slot's loads in flight at once
+  int sf;            // SpMV form (krylov_spmv_form): 3 = one row per thread, 2 = 128-row tiles,
+                     // 1 = 64-row tiles, 0 = 64-row tiles with every slot's loads in flight at once
   int ps;            // partial-sum stride: max(nb, nbs); unused entries hold zeros
   double* partial;   // [slot][ps]
   KrylovScalars* sc;
@@ -222,10 +222,9 @@ void krylov_init(const BellView& M, const TM* vals, const double* b, KVec v, Kry
 template <typename TM>
 void krylov_iteration(const BellView& M, const TM* vals, KVec v, KrylovWork w, int par, hipStream_t st, Prof* prof);
 int krylov_blocks(int64_t n);
-int krylov_spmv_blocks(int64_t n);
 int krylov_spmv_blocks(int64_t n, int form);
-int krylov_spmv_form(int64_t n);
-bool krylov_form_allowed(int64_t n, int form);
+int krylov_spmv_form(int64_t n, int n_slots);
+bool krylov_form_allowed(int64_t n, int form, int n_slots);
 int krylov_stride_max(int64_t n);
 void krylov_replace(const BellView& M, const double* vals64, const double* b, KVec v, KrylovWork w, int offer, hipStream_t st);
 void launch_spmv(const BellView& M, const double* vals, const double* x, double* y, hipStream_t st);

@@ -739,6 +739,173 @@ k_kst_all(BellView M, const TM* __restrict__ vals, KVec v, KrylovWork w, int par
   kst_body<TM, 1, true>(M, vals, v, w, par);
 }
 
+// ---- Flat SpMV form (form 3, 27-slot meshes): one row per thread over a grid of
+// ceil(n / FBS) blocks, no tile loop, every slot of the row loaded unconditionally (padding
+// slots hold zeros and point at the row's own node), so a lane has its value loads and
+// operand-pair gathers in flight together and finished waves are replaced by new blocks
+// instead of walking a run of tiles.  Each block writes one partial per dot product (the
+// stride ps is at least this grid); the consuming kernel adds all of them in index order,
+// issuing the loads before its own SpMV so that they overlap it.  (Finishing the sums in the
+// producing kernel instead, with the last block adding the partials, needs an agent-scope
+// release fence per block, an L2 write-back on gfx950: 286 us per SpMV at 64^3.)
+constexpr int FBS = 256;
+
+__device__ __forceinline__ void flat_cols(const BellView& M, int64_t r, int32_t (&j)[27]) {
+  const int64_t i = r / 3;
+#pragma unroll
+  for (int s = 0; s < 27; ++s) j[s] = M.col[(int64_t)s * M.n_nodes + i];
+}
+
+// (A u, A v) of row r for the operand pair uv = (u, v), slots in order.
+template <typename TM>
+__device__ __forceinline__ void spmv2_flat(const BellView& M, const TM* __restrict__ vals, const double2* __restrict__ uv,
+                                           int64_t r, const int32_t (&j)[27], double& a1, double& a2) {
+  const int64_t nrow = 3 * M.n_nodes;
+  a1 = a2 = 0.0;
+#pragma unroll
+  for (int s = 0; s < 27; ++s) {
+    const TM* a = vals + (3 * (int64_t)s) * nrow + r;
+    const int64_t jj = 3 * (int64_t)j[s];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double m = (double)a[c * nrow];
+      const double2 x = uv[jj + c];
+      a1 += m * x.x;
+      a2 += m * x.y;
+    }
+  }
+}
+
+// Block sums of NS values in fixed order, returned in every thread; sh: NS * (FBS / 64) doubles.
+template <int NS>
+__device__ __forceinline__ void flat_block_sums(double (&v)[NS], double* sh) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    const double t = wave_sum(v[j]);
+    if (lane == 0) sh[j * (FBS / 64) + wid] = t;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    double t = 0.0;
+#pragma unroll
+    for (int k = 0; k < FBS / 64; ++k) t += sh[j * (FBS / 64) + k];
+    v[j] = t;
+  }
+  __syncthreads();
+}
+
+// A grid smaller than the partial stride ps (small systems, where the vector kernels' grid is
+// the larger) clears the entries no block of it owns, so that a consumer summing all ps entries
+// (the x/r update) sums exactly this grid's partials.
+template <int NS>
+__device__ __forceinline__ void flat_clear_unowned(const int (&slot)[NS], double* partial, int ps) {
+  for (int i = blockIdx.x + (threadIdx.x + 1) * gridDim.x; i < ps; i += FBS * gridDim.x) {
+#pragma unroll
+    for (int j = 0; j < NS; ++j) partial[slot[j] * ps + i] = 0.0;
+  }
+}
+
+// Flat form of kpq_body (same recurrence and products).
+template <typename TM>
+__global__ void __launch_bounds__(FBS) k_kpq_flat(BellView M, const TM* __restrict__ vals, KVec v, KrylovWork w, int par) {
+  __shared__ double sh[2 * (FBS / 64)];
+  KrylovScalars* sc = w.sc;
+  const int64_t n = w.n;
+  const int64_t r = (int64_t)blockIdx.x * FBS + threadIdx.x;
+  int32_t j[27];
+  const int done = sc->done;
+  if (r < n) flat_cols(M, r, j);
+  if (done) return;  // uniform over the grid
+  const int pv = par ^ 1;
+  const double rho = sc->rho2[pv], alpha = sc->alpha2[pv], omega = sc->omega2[pv], atol2 = sc->atol2;
+  const int kprev = sc->k2[pv], maxiter = sc->maxiter;
+  double p[2] = {0.0, 0.0};  // <r,r>, <rhat,r>: the partials of the x/r update (w.nb blocks)
+  for (int i = threadIdx.x; i < w.nb; i += FBS) {
+    p[0] += w.partial[PS_RR * w.ps + i];
+    p[1] += w.partial[PS_RHR * w.ps + i];
+  }
+  double a1 = 0.0, a2 = 0.0, e0 = 0.0, e1 = 0.0;
+  double2 e2 = make_double2(0.0, 0.0);
+  if (r < n) {
+    e0 = v.rhat[r];
+    e1 = v.invd[r];
+    e2 = v.rp[r];
+    spmv2_flat<TM>(M, vals, v.rp, r, j, a1, a2);
+  }
+  flat_block_sums<2>(p, sh);
+  const double rs = p[0], rho_ = p[1];
+  int k = (omega == 0.0 || alpha == 0.0) ? -11 : kprev + 1;
+  if (rho == 0.0) k = -10;
+  const bool go = (rs > atol2) && (k < maxiter) && (k >= 0);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->k2[par] = k;
+    sc->rho2[par] = rho_;
+    sc->rs = rs;
+    sc->k = k;
+    if (!go) sc->done = 1;
+  }
+  if (!go) return;  // uniform over the grid
+  const double beta = rho_ / rho * alpha / omega;
+  double d[1] = {0.0};
+  if (r < n) {
+    const double q = __fma_rn(beta, a2, a1);
+    v.phat[r] = __fma_rn(beta, e2.y, e2.x);
+    v.q[r] = q;
+    v.rq[r].y = q * e1;
+    d[0] = e0 * q;
+  }
+  flat_block_sums<1>(d, sh);
+  if (threadIdx.x == 0) w.partial[PS_RQ * w.ps + blockIdx.x] = d[0];
+  flat_clear_unowned<1>({PS_RQ}, w.partial, w.ps);
+}
+
+// Flat form of kst_body.
+template <typename TM>
+__global__ void __launch_bounds__(FBS) k_kst_flat(BellView M, const TM* __restrict__ vals, KVec v, KrylovWork w, int par) {
+  __shared__ double sh[3 * (FBS / 64)];
+  KrylovScalars* sc = w.sc;
+  const int64_t n = w.n;
+  const int64_t r = (int64_t)blockIdx.x * FBS + threadIdx.x;
+  int32_t j[27];
+  const int done = sc->done;
+  if (r < n) flat_cols(M, r, j);
+  if (done) return;  // uniform over the grid
+  const double rho = sc->rho2[par];
+  double p[1] = {0.0};  // <rhat,q>: one partial per block of k_kpq_flat (the same grid)
+  for (int i = threadIdx.x; i < (int)gridDim.x; i += FBS) p[0] += w.partial[PS_RQ * w.ps + i];
+  double a1 = 0.0, a2 = 0.0, e0 = 0.0, e1 = 0.0;
+  double2 e2 = make_double2(0.0, 0.0);
+  if (r < n) {
+    e0 = v.r[r];
+    e1 = v.q[r];
+    e2 = v.rq[r];
+    spmv2_flat<TM>(M, vals, v.rq, r, j, a1, a2);
+  }
+  flat_block_sums<1>(p, sh);
+  const double alpha = rho / p[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->alpha2[par] = alpha;
+  double d[3] = {0.0, 0.0, 0.0};
+  if (r < n) {
+    const double tv = __fma_rn(-alpha, a2, a1);
+    const double s = e0 - alpha * e1;
+    v.s[r] = s;
+    v.shat[r] = __fma_rn(-alpha, e2.y, e2.x);
+    v.t[r] = tv;
+    d[0] = s * s;
+    d[1] = tv * s;
+    d[2] = tv * tv;
+  }
+  flat_block_sums<3>(d, sh);
+  if (threadIdx.x == 0) {
+    w.partial[PS_SS * w.ps + blockIdx.x] = d[0];
+    w.partial[PS_TS * w.ps + blockIdx.x] = d[1];
+    w.partial[PS_TT * w.ps + blockIdx.x] = d[2];
+  }
+  flat_clear_unowned<3>({PS_SS, PS_TS, PS_TT}, w.partial, w.ps);
+}
+
 // omega, x and r updates; stores rh = M r, pp = phat - omega qh and the partials of <r,r>,
 // <rhat,r>.  Scalars, the first row's operands and the partials are loaded together.
 __global__ void __launch_bounds__(KBS) k_kxr(KVec v, KrylovWork w, int par) {
@@ -862,11 +1029,17 @@ static int device_cus() {
   return cus[dev];
 }
 
-// Row groups per SpMV block: 2 when there are at least two 128-row tiles per CU, else 1.
-// SpMV form: 2 = 128-row tiles (at least two per CU), 0 = 64-row tiles with all slot loads in
-// flight (all tiles resident at three 256-thread blocks per CU), 1 = 64-row tiles otherwise.
-int krylov_spmv_form(int64_t n) {
+// SpMV form picked by size: 3 = one row per thread (27-slot meshes with at least
+// FLAT_MIN_ROWS rows), 2 = 128-row tiles (at least two per CU), 0 = 64-row tiles with all slot
+// loads in flight (all tiles resident at three 256-thread blocks per CU), 1 = 64-row tiles
+// otherwise.  Measured on the dual-phase load step (s per load step, FP64 / mixed, form 3
+// against form 2): 32^3 0.407 / 0.360 against 0.415 / 0.358, 40^3 0.888 / 0.740 against
+// 0.885 / 0.735, 48^3 1.861 / 1.447 against 1.710 / 1.308, 56^3 3.257 / 2.268 against
+// 3.311 / 2.146, 64^3 (823,875 rows) 5.126 / 3.936 against 5.328 / 3.989.
+constexpr int64_t FLAT_MIN_ROWS = 700000;
+int krylov_spmv_form(int64_t n, int n_slots) {
   const int cus = device_cus();
+  if (n_slots == 27 && n >= FLAT_MIN_ROWS) return 3;
   if ((n + SpG<2>::BLOCK_ROWS - 1) / SpG<2>::BLOCK_ROWS >= 2 * cus) return 2;
   return (n + SpG<1>::BLOCK_ROWS - 1) / SpG<1>::BLOCK_ROWS <= 3 * cus ? 0 : 1;
 }
@@ -878,24 +1051,25 @@ int krylov_spmv_form(int64_t n) {
 // matrix with 1024-thread blocks, 256 blocks took 46.3 us per BiCGSTAB iteration, 384 blocks
 // 61.7 us, 422 blocks 50.4 us).
 int krylov_spmv_blocks(int64_t n, int form) {
+  if (form == 3) return (int)std::max<int64_t>(1, (n + FBS - 1) / FBS);
   const int64_t per_cu = form == 2 ? 2 : form == 0 ? 3 : 4;
   const int64_t rows = form == 2 ? SpG<2>::BLOCK_ROWS : SpG<1>::BLOCK_ROWS;
   return (int)std::max<int64_t>(1, std::min<int64_t>((n + rows - 1) / rows, per_cu * device_cus()));
 }
-int krylov_spmv_blocks(int64_t n) { return krylov_spmv_blocks(n, krylov_spmv_form(n)); }
 
 // Whether a form may run on n rows: the all-slots form keeps row and column byte offsets in
 // 32 bits, which holds below 2^28 rows.  (Forced on the 64^3 system, 823,875 rows, where its
 // grid of three blocks per CU walks runs of 64-row tiles, it takes 153.5 us per SpMV against
 // 148 us for the 128-row form.)
-bool krylov_form_allowed(int64_t n, int form) {
+bool krylov_form_allowed(int64_t n, int form, int n_slots) {
   if (form == 1 || form == 2) return true;
+  if (form == 3) return n_slots == 27 && (n + FBS - 1) / FBS <= INT32_MAX;
   return form == 0 && n < ((int64_t)1 << 28);
 }
 
 // Partial-sum stride that holds every form's grid (the form can be set per context).
 int krylov_stride_max(int64_t n) {
-  return std::max(krylov_blocks(n), std::max(krylov_spmv_blocks(n, 1), krylov_spmv_blocks(n, 2)));
+  return std::max(std::max(krylov_blocks(n), krylov_spmv_blocks(n, 3)), std::max(krylov_spmv_blocks(n, 1), krylov_spmv_blocks(n, 2)));
 }
 
 void krylov_replace(const BellView& M, const double* vals64, const double* b, KVec v, KrylovWork w, int offer,
@@ -916,7 +1090,10 @@ static inline void timed_launch(Prof* prof, int cls, F kernel, dim3 grid, dim3 b
 template <typename TM>
 void krylov_iteration(const BellView& M, const TM* vals, KVec v, KrylovWork w, int par, hipStream_t st, Prof* prof) {
   const unsigned nb = (unsigned)w.nb, nbs = (unsigned)w.nbs;
-  if (w.sf == 2) {
+  if (w.sf == 3) {
+    timed_launch(prof, PC_SPMV, k_kpq_flat<TM>, dim3(nbs), dim3(FBS), st, M, vals, v, w, par);
+    timed_launch(prof, PC_SPMV, k_kst_flat<TM>, dim3(nbs), dim3(FBS), st, M, vals, v, w, par);
+  } else if (w.sf == 2) {
     timed_launch(prof, PC_SPMV, k_kpq<TM, 2>, dim3(nbs), dim3(SpG<2>::BS), st, M, vals, v, w, par);
     timed_launch(prof, PC_SPMV, k_kst<TM, 2>, dim3(nbs), dim3(SpG<2>::BS), st, M, vals, v, w, par);
   } else if (w.sf == 0) {

@@ -1,7 +1,9 @@
-"""The three forms of the BiCGSTAB SpMV kernels, each at a size that selects it on a 256-CU
-device: 64-row tiles with every slot's loads in flight (16^3, 14,739 dofs: 231 tiles), 64-row
-tiles one slot at a time (26^3, 59,049 dofs: 923 tiles, above three per CU), 128-row tiles
-(28^3, 73,167 dofs: 572 tiles, two per CU).  The oracle is too slow at these sizes, so each
+"""The forms of the BiCGSTAB SpMV kernels.  Three are tested at a size that selects them on a
+256-CU device: 64-row tiles with every slot's loads in flight (16^3, 14,739 dofs: 231 tiles),
+64-row tiles one slot at a time (26^3, 59,049 dofs: 923 tiles, above three per CU), 128-row
+tiles (28^3, 73,167 dofs: 572 tiles, two per CU); the fourth, one row per thread, is picked
+from 700,000 rows (the 64^3 benchmark system, covered by test_gpu_fullsize.py) and is forced
+here on small systems.  The oracle is too slow at these sizes, so each
 case checks what fixes the answer independently of the device kernels, on the first Newton
 system of the 304-like polycrystal (elastic tangent, Dirichlet values in x0), with the true
 residual computed by scipy from the exported CSR.  The recurrence stops at
@@ -109,3 +111,36 @@ def test_all_slots_form_reproduces_one_slot_form_bitwise():
     bar = 1e-10 * np.linalg.norm(bh) + 1e-9 * np.linalg.norm(bh - A @ x0h)
     assert np.linalg.norm(A @ x0f - bh) <= bar
     assert np.linalg.norm(x0f - xa) <= 1e-6 * np.linalg.norm(xa)
+
+
+@pytest.mark.parametrize("n", [4, 16, 28])
+def test_flat_form_solves_first_newton_system(n):
+    """The one-row-per-thread form (3), forced: one partial sum per 512-row block, added by the
+    consuming kernel, so its dot products run in a different order from the tile forms'.
+    FP64 and mixed solutions meet the same true-residual bar as the size-picked form and land
+    within 1e-6 of it.  Iteration counts are not compared: BiCGSTAB's count moves with the
+    summation order by up to ~10% on these systems in FP64 (28^3: 353 against 319) and more in
+    the mixed form (16^3: 223 against 192); the oracle comparison of the count against its own
+    rounding spread is in test_gpu_parity.py."""
+    import cpfem_amd as C
+    from cpfem_amd import _native as N
+    from cpfem_amd.solver import krylov_solve
+    prob, mat, b, x0 = _first_system(n)
+    A = C.BlockEllMatrix(prob.ctx, mat).to_scipy()
+    bh, x0h = b.cpu().numpy(), x0.cpu().numpy()
+    bar = 1e-10 * np.linalg.norm(bh) + 1e-9 * np.linalg.norm(bh - A @ x0h)
+    for prec in ("fp64", "mixed"):
+        N.check(prob.ctx._lib.cpfem_set_spmv_form(prob.ctx.h, -1))
+        xa = krylov_solve(prob, mat, b, x0, dict(precision=prec)).cpu().numpy()
+        ka = prob.solver_info[-1]["iterations"]
+        N.check(prob.ctx._lib.cpfem_set_spmv_form(prob.ctx.h, 3))
+        assert _form(prob) == 3
+        xf = krylov_solve(prob, mat, b, x0, dict(precision=prec)).cpu().numpy()
+        kf = prob.solver_info[-1]["iterations"]
+        res = np.linalg.norm(A @ xf - bh)
+        err = np.linalg.norm(xf - xa) / np.linalg.norm(xa)
+        print(f"n={n} {prec}: picked form {ka} iterations, flat form {kf}, |Ax - b| {res:.2e} (bar {bar:.2e}), "
+              f"|x - x_picked|/|x| {err:.2e}", flush=True)
+        assert res <= bar, (prec, res, bar)
+        assert err < 1e-6, (prec, err)
+    N.check(prob.ctx._lib.cpfem_set_spmv_form(prob.ctx.h, -1))
