@@ -219,6 +219,9 @@ __device__ __forceinline__ void get_sums(const double* partial, const int (&slot
   double s[NSL];
 #pragma unroll
   for (int j = 0; j < NSL; ++j) s[j] = 0.0;
+  // unrolled so that several iterations' loads are in flight together (the flat SpMV form
+  // leaves thousands of partials per sum); the additions keep their order
+#pragma unroll 4
   for (int i = threadIdx.x; i < ps; i += BS) {
 #pragma unroll
     for (int j = 0; j < NSL; ++j) s[j] += partial[slot[j] * ps + i];
@@ -874,6 +877,7 @@ __global__ void __launch_bounds__(FBS) k_kst_flat(BellView M, const TM* __restri
   if (done) return;  // uniform over the grid
   const double rho = sc->rho2[par];
   double p[1] = {0.0};  // <rhat,q>: one partial per block of k_kpq_flat (the same grid)
+#pragma unroll 4
   for (int i = threadIdx.x; i < (int)gridDim.x; i += FBS) p[0] += w.partial[PS_RQ * w.ps + i];
   double a1 = 0.0, a2 = 0.0, e0 = 0.0, e1 = 0.0;
   double2 e2 = make_double2(0.0, 0.0);
