@@ -194,7 +194,9 @@ struct KrylovWork {
 // iteration keeps rh = M r, qh = M q and pp = M (p - omega q), so that the next phat =
 // rh + beta pp and shat = rh - alpha qh can be formed inside the SpMV kernels.  They are
 // stored as interleaved pairs rp = (rh, pp) and rq = (rh, qh), so that each SpMV gathers its
-// two operand vectors with one 16-byte load per column entry.
+// two operand vectors with one 16-byte load per column entry.  rh is written into rp (x/r
+// update, init, residual replacement); the first SpMV kernel writes rq whole, with its own
+// row's rh, when it forms qh.
 struct KVec {
   double *x, *r, *rhat, *q, *phat, *s, *shat, *t;
   double2 *rp, *rq;

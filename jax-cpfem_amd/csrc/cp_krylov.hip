@@ -254,7 +254,6 @@ __global__ void __launch_bounds__(KBS) k_kinit(BellView M, const TM* __restrict_
     v.rhat[i] = r;
     const double rh = r * v.invd[i];
     v.rp[i] = make_double2(rh, 0.0);  // pp = M (p0 - omega0 q0) = 0 with p0 = q0 = r0, omega0 = 1
-    v.rq[i].x = rh;
     rr += r * r;
     bb += b[i] * b[i];
   }
@@ -641,7 +640,7 @@ __device__ __forceinline__ void kpq_body(const BellView& M, const TM* __restrict
       const double q = __fma_rn(beta, a2, a1);
       v.phat[r] = __fma_rn(beta, e2.y, e2.x);
       v.q[r] = q;
-      v.rq[r].y = q * e1;
+      v.rq[r] = make_double2(e2.x, q * e1);  // (rh, qh) for k_kst, whole 16-byte pairs
       d += e0 * q;
     }
     if (++tile >= tiles.end) break;  // uniform over the block
@@ -856,7 +855,7 @@ __global__ void __launch_bounds__(FBS) k_kpq_flat(BellView M, const TM* __restri
     const double q = __fma_rn(beta, a2, a1);
     v.phat[r] = __fma_rn(beta, e2.y, e2.x);
     v.q[r] = q;
-    v.rq[r].y = q * e1;
+    v.rq[r] = make_double2(e2.x, q * e1);  // (rh, qh) for k_kst_flat, whole 16-byte pairs
     d[0] = e0 * q;
   }
   flat_block_sums<1>(d, sh);
@@ -947,7 +946,6 @@ __global__ void __launch_bounds__(KBS) k_kxr(KVec v, KrylovWork w, int par) {
     v.r[i] = r;
     const double rh = r * invd;
     v.rp[i] = make_double2(rh, __fma_rn(-omega, qh, ph));
-    v.rq[i].x = rh;
     rr += r * r;
     rhr += rhat * r;
   }
@@ -977,7 +975,6 @@ __global__ void __launch_bounds__(KBS) k_kreplace(BellView M, const double* __re
     v.r[i] = r;
     const double rh = r * v.invd[i];
     v.rp[i].x = rh;
-    v.rq[i].x = rh;
     rr += r * r;
     rhr += v.rhat[i] * r;
   }
