@@ -41,12 +41,15 @@ CPFEM_HD Dual& operator+=(Dual& a, Dual b) {
 CPFEM_HD Dual dabs(Dual a) { return a.v < 0.0 ? Dual{-a.v, -a.d} : (a.v > 0.0 ? a : Dual{0.0, 0.0}); }
 // x^y for x >= 0 with the conventions of lax.pow's jvp: d/dx = y x^(y-1) (0 at x = 0 for y > 1),
 // d/dy = x^y log x (0 at x = 0).
+// For x > 0 the value is exp(y ln x) with the logarithm shared by both derivative terms
+// (pow_pos of cp_point.hpp: the library pow costs ~60% more instructions).
 CPFEM_HD Dual dpow(Dual x, Dual y) {
-  const double pv = pow(x.v, y.v);
-  double d = 0.0;
-  if (x.v > 0.0) d = pv * (y.d * log(x.v) + y.v * x.d / x.v);
-  else if (y.v == 1.0) d = x.d;
-  return {pv, d};
+  if (x.v > 0.0) {
+    const double lx = log(x.v);
+    const double pv = exp(y.v * lx);
+    return {pv, pv * (y.d * lx + y.v * x.d / x.v)};
+  }
+  return {pow(x.v, y.v), y.v == 1.0 ? x.d : 0.0};
 }
 CPFEM_HD double sgnv(Dual a) { return sgn(a.v); }
 
