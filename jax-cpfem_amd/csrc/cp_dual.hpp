@@ -41,12 +41,12 @@ CPFEM_HD Dual& operator+=(Dual& a, Dual b) {
 CPFEM_HD Dual dabs(Dual a) { return a.v < 0.0 ? Dual{-a.v, -a.d} : (a.v > 0.0 ? a : Dual{0.0, 0.0}); }
 // x^y for x >= 0 with the conventions of lax.pow's jvp: d/dx = y x^(y-1) (0 at x = 0 for y > 1),
 // d/dy = x^y log x (0 at x = 0).
-// For x > 0 the value is exp(y ln x) with the logarithm shared by both derivative terms
-// (pow_pos of cp_point.hpp: the library pow costs ~60% more instructions).
+// For x > 0 the value is e^(y ln x) (ln_pos, exp_arg of cp_point.hpp) with the logarithm
+// shared by both derivative terms.
 CPFEM_HD Dual dpow(Dual x, Dual y) {
   if (x.v > 0.0) {
-    const double lx = log(x.v);
-    const double pv = exp(y.v * lx);
+    const double lx = ln_pos(x.v);
+    const double pv = exp_arg(y.v * lx);
     return {pv, pv * (y.d * lx + y.v * x.d / x.v)};
   }
   return {pow(x.v, y.v), y.v == 1.0 ? x.d : 0.0};
@@ -56,7 +56,7 @@ CPFEM_HD double sgnv(Dual a) { return sgn(a.v); }
 CPFEM_HD double dval(double a) { return a; }
 CPFEM_HD double dval(Dual a) { return a.v; }
 CPFEM_HD double dabs(double a) { return fabs(a); }
-CPFEM_HD double dpow(double x, double y) { return pow(x, y); }
+CPFEM_HD double dpow(double x, double y) { return x > 0.0 ? pow_pos(x, y) : pow(x, y); }
 CPFEM_HD double sgnv(double a) { return sgn(a); }
 
 template <typename T>

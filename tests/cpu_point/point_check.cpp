@@ -137,3 +137,12 @@ extern "C" int cpu_point_vjp(int n_slip, int64_t Q, const double* H, const doubl
     return 1;
   return 0;
 }
+
+// The kernel's logarithm, exponential and power (cp_point.hpp) on n arguments.
+extern "C" void cpu_fastmath(int64_t n, const double* x, const double* p, double* ln, double* ex, double* pw) {
+  for (int64_t i = 0; i < n; ++i) {
+    ln[i] = cpfem::ln_pos(x[i]);
+    ex[i] = cpfem::exp_arg(x[i]);
+    pw[i] = cpfem::pow_pos(x[i], p[i]);
+  }
+}

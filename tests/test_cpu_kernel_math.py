@@ -100,3 +100,53 @@ def test_nonfinite_history_sets_the_nonfinite_status(pointlib):
     assert o["st"][7] & 2 and np.isnan(o["P"][7]).all()
     others = np.arange(Q) != 7
     assert np.all(o["st"][others] == 0) and np.isfinite(o["P"][others]).all()
+
+
+def test_kernel_log_exp_pow_against_c_library(pointlib):
+    """The kernel's own ln, exp and power (cp_point.hpp: series on a range-reduced argument)
+    against the C library through numpy, in units of the result's last place: ln over
+    log-uniform arguments 1e-300..1e300 and near 1, exp over its whole finite range, the power
+    for the slip-rate exponents 1/m (m = 0.01..0.5) on |tau/g| in 1e-6..3 within
+    (4 |p ln x| + 8) 2^-52 relative (the logarithm's few units in the last place, amplified by
+    the exponent); special values as the C library gives them."""
+    rng = np.random.default_rng(11)
+    lib = pointlib
+    lib.cpu_fastmath.argtypes = [ctypes.c_int64] + [ctypes.c_void_p] * 5
+
+    def run(x, p):
+        x = np.ascontiguousarray(x, dtype=float)
+        p = np.ascontiguousarray(p, dtype=float)
+        out = [np.zeros_like(x) for _ in range(3)]
+        lib.cpu_fastmath(x.size, *[a.ctypes.data_as(ctypes.c_void_p) for a in (x, p, *out)])
+        return out
+
+    def ulps(a, b):
+        return np.abs(a - b) / np.spacing(np.abs(b))
+
+    x = np.concatenate([10.0 ** rng.uniform(-300, 300, 20000), 1.0 + rng.uniform(-0.3, 0.3, 20000),
+                        [np.sqrt(0.5), np.sqrt(2.0), 0.5, 1.0, 2.0, 4.9e-324, 2.2250738585072014e-308]])
+    ln, _, _ = run(x, np.ones_like(x))
+    ref = np.log(x)
+    nz = ref != 0.0
+    assert ulps(ln[nz], ref[nz]).max() <= 4, ulps(ln[nz], ref[nz]).max()
+    assert ln[x == 1.0][0] == 0.0
+    xe = np.concatenate([rng.uniform(-708.0, 709.7, 40000), rng.uniform(-1.0, 1.0, 10000), [0.0, 1.0, -1.0]])
+    _, ex, _ = run(xe, np.ones_like(xe))
+    assert ulps(ex, np.exp(xe)).max() <= 4, ulps(ex, np.exp(xe)).max()
+    tau = 10.0 ** rng.uniform(-6, np.log10(3.0), 40000)
+    pexp = 1.0 / rng.uniform(0.01, 0.5, 40000)
+    _, _, pw = run(tau, pexp)
+    refp = tau ** pexp
+    ok = refp > 1e-300
+    relp = np.abs(pw[ok] - refp[ok]) / refp[ok]
+    bound = (4.0 * np.abs(pexp[ok] * np.log(tau[ok])) + 8) * 2.0 ** -52
+    assert (relp <= bound).all(), (relp / bound).max()
+    sp = np.array([0.0, -0.0, np.inf, np.nan, -1.0, 800.0, -800.0])
+    ln_s, ex_s, pw_s = run(sp, np.full(sp.shape, 20.0))
+    with np.errstate(all="ignore"):
+        np.testing.assert_array_equal(ln_s[:3], np.log(sp[:3]))
+        assert np.isnan(ln_s[3]) and np.isnan(ln_s[4])
+        np.testing.assert_array_equal(ex_s[[0, 1, 2, 5, 6]], np.exp(sp[[0, 1, 2, 5, 6]]))
+        assert np.isnan(ex_s[3])
+        np.testing.assert_array_equal(pw_s[[0, 1, 2]], np.power(sp[[0, 1, 2]], 20.0))
+        assert np.isnan(pw_s[3])
