@@ -281,8 +281,11 @@ __global__ void __launch_bounds__(QBS) k_element_qp_park(Geom G, const double* _
 constexpr int EK_CELLS = 16;
 constexpr int EK_BS = 8 * EK_CELLS;  // threads = points per block
 
-__global__ void __launch_bounds__(EK_BS) k_element_K(Geom G, const double* __restrict__ wP, const double* __restrict__ wA,
-                                                    double* __restrict__ KE, double* __restrict__ RE) {
+// Two waves per SIMD: 256 VGPRs with 12 B of spills (one wave per SIMD at 258 registers
+// without the bound).
+__global__ void __launch_bounds__(EK_BS) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_element_K(Geom G, const double* __restrict__ wP, const double* __restrict__ wA, double* __restrict__ KE,
+            double* __restrict__ RE) {
   __shared__ double sP[9][EK_BS];
   __shared__ double sA[27][EK_BS];
   const int64_t q0 = (int64_t)blockIdx.x * EK_BS;  // first point of the block
