@@ -78,6 +78,8 @@ def spmv_kernel_names(form, tm):
     """Names of the BiCGSTAB SpMV kernel pair of an SpMV form (as the library reports it,
     cpfem_krylov_config): 3 = one row per thread, 2 = 128-row tiles, 1 = 64-row tiles, 0 =
     64-row tiles with every slot's loads in flight."""
+    if form == 3 and tm == "double":  # the one-operand recurrence of the FP64 flat form
+        return ["cpfem::k_kpq_flat1", "cpfem::k_kst_flat1"]
     return {3: [f"cpfem::k_kpq_flat<{tm}>", f"cpfem::k_kst_flat<{tm}>"],
             2: [f"cpfem::k_kpq<{tm}, 2>", f"cpfem::k_kst<{tm}, 2>"],
             1: [f"cpfem::k_kpq<{tm}, 1>", f"cpfem::k_kst<{tm}, 1>"],

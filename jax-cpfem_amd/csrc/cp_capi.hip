@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -538,6 +539,7 @@ static int run_bicgstab(cpfem_ctx* c, hipStream_t st, const KBufs& kb, const TM*
   w.sf = c->krylov_form();
   w.nbs = krylov_spmv_blocks(c->n_dofs, w.sf);
   w.ps = std::max(w.nb, w.nbs);
+  w.single = (w.sf == 3 && std::is_same<TM, double>::value) ? 1 : 0;
   w.partial = kb.partial;
   w.sc = kb.sc;
   KVec v = kvec(const_cast<double**>(kb.v), x);

@@ -186,6 +186,7 @@ struct KrylovWork {
   int sf;            // SpMV form (krylov_spmv_form): 3 = one row per thread, 2 = 128-row tiles,
                      // 1 = 64-row tiles, 0 = 64-row tiles with every slot's loads in flight at once
   int ps;            // partial-sum stride: max(nb, nbs); unused entries hold zeros
+  int single;        // one-operand recurrence (FP64 flat form): phat formed by the x/r update
   double* partial;   // [slot][ps]
   KrylovScalars* sc;
 };

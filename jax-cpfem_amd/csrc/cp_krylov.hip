@@ -26,6 +26,7 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "cp_internal.hpp"
 
@@ -183,7 +184,10 @@ __global__ void __launch_bounds__(KBS) k_inv_diag(BellView M, const double* __re
 // so every block sees the bitwise-identical scalar.  Scalars that carry over to the next
 // iteration live in two parity slots: iteration j writes slot (j & 1) and reads slot
 // ((j + 1) & 1), which the host passes as `par`.
-enum { PS_RR = 0, PS_RHR, PS_RQ, PS_SS, PS_TS, PS_TT, PS_BB, PS_COUNT };
+// PS_RS and PS_RT (<rhat,s>, <rhat,t>: the one-operand recurrence) share PS_BB's slot
+// (<b,b>, read once by k_kinit2) and the last one.
+enum { PS_RR = 0, PS_RHR, PS_RQ, PS_SS, PS_TS, PS_TT, PS_BB, PS_RT, PS_COUNT };
+constexpr int PS_RS = PS_BB;
 
 // Writes this block's NSL sums into entry `own` (a permutation of the block index over the
 // grid; the block index by default) of their slots; a grid smaller than the stride ps also
@@ -253,7 +257,12 @@ __global__ void __launch_bounds__(KBS) k_kinit(BellView M, const TM* __restrict_
     v.r[i] = r;
     v.rhat[i] = r;
     const double rh = r * v.invd[i];
-    v.rp[i] = make_double2(rh, 0.0);  // pp = M (p0 - omega0 q0) = 0 with p0 = q0 = r0, omega0 = 1
+    if (w.single) {  // phat0 = M p0 = rh; rh itself in the first half of rp's storage
+      v.phat[i] = rh;
+      reinterpret_cast<double*>(v.rp)[i] = rh;
+    } else {
+      v.rp[i] = make_double2(rh, 0.0);  // pp = M (p0 - omega0 q0) = 0 with p0 = q0 = r0, omega0 = 1
+    }
     rr += r * r;
     bb += b[i] * b[i];
   }
@@ -778,6 +787,22 @@ __device__ __forceinline__ void spmv2_flat(const BellView& M, const TM* __restri
   }
 }
 
+// A u of row r for one stored operand u, slots in order.
+template <typename TM>
+__device__ __forceinline__ double spmv1_flat(const BellView& M, const TM* __restrict__ vals, const double* __restrict__ u,
+                                             int64_t r, const int32_t (&j)[27]) {
+  const int64_t nrow = 3 * M.n_nodes;
+  double a = 0.0;
+#pragma unroll
+  for (int s = 0; s < 27; ++s) {
+    const TM* m = vals + (3 * (int64_t)s) * nrow + r;
+    const int64_t jj = 3 * (int64_t)j[s];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) a += (double)m[c * nrow] * u[jj + c];
+  }
+  return a;
+}
+
 // Block sums of NS values in fixed order, returned in every thread; sh: NS * (FBS / 64) doubles.
 template <int NS>
 __device__ __forceinline__ void flat_block_sums(double (&v)[NS], double* sh) {
@@ -907,6 +932,160 @@ __global__ void __launch_bounds__(FBS) k_kst_flat(BellView M, const TM* __restri
     w.partial[PS_TT * w.ps + blockIdx.x] = d[2];
   }
   flat_clear_unowned<3>({PS_SS, PS_TS, PS_TT}, w.partial, w.ps);
+}
+
+// ---- One-operand recurrence (FP64 values, flat form).  The x/r update forms the next
+// phat = rh + beta (phat - omega qh) itself, with beta from <rhat, r_new> = <rhat,s> -
+// omega <rhat,t> (sums the second SpMV kernel adds for it: the improved BiCGSTAB of Yang and
+// Brent, equal to JAX's recurrence in exact arithmetic, alpha and the convergence test still
+// from the directly summed <rhat,r> and <r,r>), so the first SpMV gathers one 8-byte operand
+// per column entry instead of the (rh, pp) pair.  rh is kept in the first half of rp's
+// storage.  (The mixed form keeps the pair: its residual replacements change r between the
+// x/r update and the first SpMV.)
+__global__ void __launch_bounds__(FBS) k_kpq_flat1(BellView M, const double* __restrict__ vals, KVec v, KrylovWork w,
+                                                   int par) {
+  __shared__ double sh[2 * (FBS / 64)];
+  KrylovScalars* sc = w.sc;
+  const int64_t n = w.n;
+  const int64_t r = (int64_t)blockIdx.x * FBS + threadIdx.x;
+  int32_t j[27];
+  const int done = sc->done;
+  if (r < n) flat_cols(M, r, j);
+  if (done) return;  // uniform over the grid
+  const int pv = par ^ 1;
+  const double rho = sc->rho2[pv], alpha = sc->alpha2[pv], omega = sc->omega2[pv], atol2 = sc->atol2;
+  const int kprev = sc->k2[pv], maxiter = sc->maxiter;
+  double p[2] = {0.0, 0.0};  // <r,r>, <rhat,r>: the partials of the x/r update (w.nb blocks)
+  for (int i = threadIdx.x; i < w.nb; i += FBS) {
+    p[0] += w.partial[PS_RR * w.ps + i];
+    p[1] += w.partial[PS_RHR * w.ps + i];
+  }
+  const double* __restrict__ rhv = reinterpret_cast<const double*>(v.rp);
+  double a = 0.0, e0 = 0.0, e1 = 0.0, e2 = 0.0;
+  if (r < n) {
+    e0 = v.rhat[r];
+    e1 = v.invd[r];
+    e2 = rhv[r];
+    a = spmv1_flat<double>(M, vals, v.phat, r, j);
+  }
+  flat_block_sums<2>(p, sh);
+  const double rs = p[0], rho_ = p[1];
+  int k = (omega == 0.0 || alpha == 0.0) ? -11 : kprev + 1;
+  if (rho == 0.0) k = -10;
+  const bool go = (rs > atol2) && (k < maxiter) && (k >= 0);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->k2[par] = k;
+    sc->rho2[par] = rho_;
+    sc->rs = rs;
+    sc->k = k;
+    if (!go) sc->done = 1;
+  }
+  if (!go) return;  // uniform over the grid
+  double d[1] = {0.0};
+  if (r < n) {
+    v.q[r] = a;
+    v.rq[r] = make_double2(e2, a * e1);  // (rh, qh) for the second SpMV
+    d[0] = e0 * a;
+  }
+  flat_block_sums<1>(d, sh);
+  if (threadIdx.x == 0) w.partial[PS_RQ * w.ps + blockIdx.x] = d[0];
+  flat_clear_unowned<1>({PS_RQ}, w.partial, w.ps);
+}
+
+// k_kst_flat with the partials of <rhat,s> and <rhat,t> as well.
+__global__ void __launch_bounds__(FBS) k_kst_flat1(BellView M, const double* __restrict__ vals, KVec v, KrylovWork w,
+                                                   int par) {
+  __shared__ double sh[5 * (FBS / 64)];
+  KrylovScalars* sc = w.sc;
+  const int64_t n = w.n;
+  const int64_t r = (int64_t)blockIdx.x * FBS + threadIdx.x;
+  int32_t j[27];
+  const int done = sc->done;
+  if (r < n) flat_cols(M, r, j);
+  if (done) return;  // uniform over the grid
+  const double rho = sc->rho2[par];
+  double p[1] = {0.0};  // <rhat,q>: one partial per block of k_kpq_flat1 (the same grid)
+#pragma unroll 4
+  for (int i = threadIdx.x; i < (int)gridDim.x; i += FBS) p[0] += w.partial[PS_RQ * w.ps + i];
+  double a1 = 0.0, a2 = 0.0, e0 = 0.0, e1 = 0.0, e3 = 0.0;
+  double2 e2 = make_double2(0.0, 0.0);
+  if (r < n) {
+    e0 = v.r[r];
+    e1 = v.q[r];
+    e2 = v.rq[r];
+    e3 = v.rhat[r];
+    spmv2_flat<double>(M, vals, v.rq, r, j, a1, a2);
+  }
+  flat_block_sums<1>(p, sh);
+  const double alpha = rho / p[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->alpha2[par] = alpha;
+  double d[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (r < n) {
+    const double tv = __fma_rn(-alpha, a2, a1);
+    const double s = e0 - alpha * e1;
+    v.s[r] = s;
+    v.shat[r] = __fma_rn(-alpha, e2.y, e2.x);
+    v.t[r] = tv;
+    d[0] = s * s;
+    d[1] = tv * s;
+    d[2] = tv * tv;
+    d[3] = e3 * s;
+    d[4] = e3 * tv;
+  }
+  flat_block_sums<5>(d, sh);
+  if (threadIdx.x == 0) {
+    w.partial[PS_SS * w.ps + blockIdx.x] = d[0];
+    w.partial[PS_TS * w.ps + blockIdx.x] = d[1];
+    w.partial[PS_TT * w.ps + blockIdx.x] = d[2];
+    w.partial[PS_RS * w.ps + blockIdx.x] = d[3];
+    w.partial[PS_RT * w.ps + blockIdx.x] = d[4];
+  }
+  flat_clear_unowned<5>({PS_SS, PS_TS, PS_TT, PS_RS, PS_RT}, w.partial, w.ps);
+}
+
+// k_kxr for the one-operand recurrence: also forms the next phat.
+__global__ void __launch_bounds__(KBS) k_kxr1(KVec v, KrylovWork w, int par) {
+  __shared__ double sh[5 * (KBS / 64)];
+  KrylovScalars* sc = w.sc;
+  const int done = sc->done;
+  const double alpha = sc->alpha2[par], rho = sc->rho2[par], atol2 = sc->atol2;
+  const int64_t i0 = (int64_t)blockIdx.x * KBS + threadIdx.x;
+  double x = 0, ph = 0, shv = 0, s = 0, t = 0, qh = 0, rhat = 0, invd = 0;
+  if (i0 < w.n) {  // issued before the done test (one round trip for both)
+    x = v.x[i0]; ph = v.phat[i0]; shv = v.shat[i0]; s = v.s[i0]; t = v.t[i0];
+    qh = v.rq[i0].y; rhat = v.rhat[i0]; invd = v.invd[i0];
+  }
+  if (done) return;
+  double sums[5];
+  get_sums<KBS, 5>(w.partial, {PS_SS, PS_TS, PS_TT, PS_RS, PS_RT}, w.ps, sh, sums);
+  const double ss = sums[0];
+  const double omega = sums[1] / sums[2];
+  const double beta = (sums[3] - omega * sums[4]) / rho * alpha / omega;  // of the next iteration
+  const bool early = ss < atol2;
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->omega2[par] = omega;
+  double* __restrict__ rhv = reinterpret_cast<double*>(v.rp);
+  double rr = 0.0, rhr = 0.0;
+  for (int64_t i = i0; i < w.n; i += (int64_t)w.nb * KBS) {
+    if (i != i0) {
+      x = v.x[i]; ph = v.phat[i]; shv = v.shat[i]; s = v.s[i]; t = v.t[i];
+      qh = v.rq[i].y; rhat = v.rhat[i]; invd = v.invd[i];
+    }
+    double r;
+    if (early) {
+      v.x[i] = x + alpha * ph;
+      r = s;
+    } else {
+      v.x[i] = x + (alpha * ph + omega * shv);
+      r = s - omega * t;
+    }
+    v.r[i] = r;
+    const double rh = r * invd;
+    rhv[i] = rh;
+    v.phat[i] = __fma_rn(beta, __fma_rn(-omega, qh, ph), rh);
+    rr += r * r;
+    rhr += rhat * r;
+  }
+  put_partials<KBS, 2>({rr, rhr}, {PS_RR, PS_RHR}, w.partial, w.ps, sh);
 }
 
 // omega, x and r updates; stores rh = M r, pp = phat - omega qh and the partials of <r,r>,
@@ -1092,6 +1271,14 @@ template <typename TM>
 void krylov_iteration(const BellView& M, const TM* vals, KVec v, KrylovWork w, int par, hipStream_t st, Prof* prof) {
   const unsigned nb = (unsigned)w.nb, nbs = (unsigned)w.nbs;
   if (w.sf == 3) {
+    if constexpr (std::is_same<TM, double>::value) {
+      if (w.single) {
+        timed_launch(prof, PC_SPMV, k_kpq_flat1, dim3(nbs), dim3(FBS), st, M, vals, v, w, par);
+        timed_launch(prof, PC_SPMV, k_kst_flat1, dim3(nbs), dim3(FBS), st, M, vals, v, w, par);
+        timed_launch(prof, PC_KVEC, k_kxr1, dim3(nb), dim3(KBS), st, v, w, par);
+        return;
+      }
+    }
     timed_launch(prof, PC_SPMV, k_kpq_flat<TM>, dim3(nbs), dim3(FBS), st, M, vals, v, w, par);
     timed_launch(prof, PC_SPMV, k_kst_flat<TM>, dim3(nbs), dim3(FBS), st, M, vals, v, w, par);
   } else if (w.sf == 2) {
