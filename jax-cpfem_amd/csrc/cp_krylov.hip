@@ -22,7 +22,8 @@
 // fused into the kernel producing its operand as per-block partial sums, which the consuming
 // kernel adds in a fixed order (bitwise reproducible run to run, no atomics).  All scalars
 // stay on the device; each kernel returns at once when the device-side `done` flag is set,
-// so the host enqueues iterations in batches and polls rarely.
+// so the host enqueues iterations in batches and polls rarely.  (On the largest systems in
+// FP64 a one-operand variant stores phat instead: k_kpq_flat1 below.)
 #include <hip/hip_ext.h>
 
 #include <algorithm>
