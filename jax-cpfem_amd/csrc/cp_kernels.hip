@@ -235,10 +235,10 @@ __global__ void __launch_bounds__(QBS) k_point_tangent(Geom G, const double* __r
   });
 }
 
-// The parked points of k_element_qp, one queue entry per lane: each point's stress iterate is
+// The parked points of k_element_res, one queue entry per lane: each point's stress iterate is
 // restored, its residual re-evaluated there and its Newton iterations continued to
 // convergence.  The grid covers the longest possible queue; blocks past the queue's length
-// (final when this kernel starts: k_element_qp ran before it on the same stream) exit at once,
+// (final when this kernel starts: k_element_res ran before it on the same stream) exit at once,
 // so only as many blocks as parked points run, dispatched as earlier blocks retire.  (Looping
 // each wave over the queue instead costs this kernel 64 spilled registers per lane.)
 template <int N>
