@@ -47,17 +47,22 @@ def log(msg):
 
 
 # ------------------------------------------------------------------------------ byte / flop models
-def spmv_bytes(prob, value_bytes=8):
+def spmv_bytes(prob, value_bytes=8, computed_columns=False):
     """Algorithmic HBM bytes of one Krylov SpMV launch (k_kpq / k_kst, equal counts): the
     block values (value_bytes each), one int32 column index per 3x3 block (plus the slot
     counts, read unless every row is padded to 27 slots; padding is not counted), the two FP64
     vectors the operand is formed from (read once each), and five FP64 row vectors (k_kpq:
     reads rhat and the Jacobi vector, writes phat, q, M q; k_kst: reads r, q, writes s, shat,
-    t)."""
+    t).  computed_columns: the one-row-per-thread form on a box-numbered mesh computes the
+    columns of the nodes off the box's faces, so only the face nodes' indices count."""
     nslot, _ = prob.ctx.pattern()
     nnzb = int(nslot.sum())
     n = prob.ctx.n_dofs
     idx = 4 * nnzb + (0 if prob.ctx.n_slots == 27 else 4 * prob.ctx.n_nodes)
+    if computed_columns:
+        nx, ny, nz = prob.ctx.box_numbering()
+        if nx > 0:
+            idx -= 4 * 27 * (nx - 2) * (ny - 2) * (nz - 2)
     return 9 * value_bytes * nnzb + idx + 7 * 8 * n, nnzb
 
 
@@ -359,7 +364,9 @@ def main():
                     help="park the point kernel's lanes when at most this many of a wave still iterate "
                          "(0: never; -1: the library's default)")
     ap.add_argument("--spmv-form", type=int, default=-1,
-                    help="force the BiCGSTAB SpMV form (0-2; -1: the library's choice by size)")
+                    help="force the BiCGSTAB SpMV form (0-3; -1: the library's choice by size)")
+    ap.add_argument("--no-box-columns", action="store_true",
+                    help="load every column in the one-row-per-thread SpMV (default: computed on box meshes)")
     ap.add_argument("--history-adjoint-mesh", type=int, default=16)
     ap.add_argument("--copy-calibration", action="store_true", help="also run a plain copy kernel (counter scale)")
     ap.add_argument("--direct", action="store_true",
@@ -392,6 +399,8 @@ def main():
         prob.ctx.set_point_park(args.park)
     if args.spmv_form >= 0:
         prob.ctx.set_spmv_form(args.spmv_form)
+    if args.no_box_columns:
+        prob.ctx.set_box_columns(False)
     log(f"[bench] context built in {time.perf_counter() - t0:.1f}s")
     params = prob.internal_vars
     sol = [torch.zeros(prob.fes[0].num_total_nodes, 3, dtype=torch.float64, device="cuda")]
@@ -435,10 +444,10 @@ def main():
     Q = w.n_qp
     value = Q * K / T
     vb = 4 if args.precision == "mixed" else 8
-    sb, nnzb = spmv_bytes(prob, value_bytes=vb)
+    form = prob.ctx.krylov_config()[0]
+    sb, nnzb = spmv_bytes(prob, value_bytes=vb, computed_columns=form == 3 and not args.no_box_columns)
     cus = torch.cuda.get_device_properties(0).multi_processor_count
     tm = "float" if vb == 4 else "double"
-    form = prob.ctx.krylov_config()[0]
     names = spmv_kernel_names(form, tm)
     tkey = f"c{args.config}_{n_mesh}"
     point_ms = lambda: (kp["element_qp"]["est_total_ms"] + kp["element_qp_park"]["est_total_ms"]

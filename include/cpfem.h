@@ -178,6 +178,15 @@ int cpfem_solve_direct(cpfem_ctx* ctx, void* stream, const double* mat, const do
    ip[old node] = new node; *bandwidth = max |ip[i] - ip[j]| over coupled nodes. */
 int cpfem_band_order(int64_t n_nodes, const int32_t* col, const int32_t* nslot, int32_t* ip, int32_t* bandwidth);
 
+/* Box numbering of a block-ELL pattern, detected on the host (no device needed; col/nslot as
+   cpfem_pattern returns them): dims = (nx, ny, nz) nodes if the nodes of an nx x ny x nz box
+   are numbered i = ix + nx iy + nx ny iz and every node off the box's faces has exactly the
+   27 columns i + dz nx ny + dy nx + dx (dx, dy, dz in {-1, 0, 1}); (0, 0, 0) otherwise.  A
+   context whose pattern has it lets the one-row-per-thread SpMV compute those nodes' columns
+   instead of loading them.  No counterpart in the reference (a property of the mesh's node
+   numbering; the reference's box meshes have it). */
+int cpfem_box_numbering(int64_t n_nodes, const int32_t* col, const int32_t* nslot, int64_t* dims);
+
 /* y = A x on the block-ELL matrix (device). */
 int cpfem_spmv(cpfem_ctx* ctx, void* stream, const double* mat, const double* x, double* y);
 
@@ -202,6 +211,11 @@ int cpfem_krylov_config(const cpfem_ctx* ctx, int32_t* out);
    (CPFEM_EBADARG) on systems of 2^28 rows or more, form 3 on meshes whose rows have other
    than 27 node slots.  No counterpart in the reference. */
 int cpfem_set_spmv_form(cpfem_ctx* ctx, int32_t form);
+/* Computed node columns of the one-row-per-thread SpMV on a box-numbered pattern (see
+   cpfem_box_numbering): 1 = on (the default), 0 = every column loaded.  The products are
+   bitwise the same either way; for tests and measurements.  No counterpart in the
+   reference. */
+int cpfem_set_box_columns(cpfem_ctx* ctx, int32_t enable);
 
 /* Work counters of the assembly's point kernel, summed over its launches since the last
    reset: out[0] = local Newton iterations, out[1] = residual evaluations, out[2] = parked

@@ -82,6 +82,7 @@ _SIGS = {
     "cpfem_spmv": (ctypes.c_int, [vp, vp, vp, vp, vp]),
     "cpfem_solve_direct": (ctypes.c_int, [vp, vp, vp, vp, vp, ctypes.c_int32, ctypes.POINTER(CpfemSolveInfo)]),
     "cpfem_band_order": (ctypes.c_int, [ctypes.c_int64, vp, vp, vp, vp]),
+    "cpfem_box_numbering": (ctypes.c_int, [ctypes.c_int64, vp, vp, vp]),
     "cpfem_norm": (ctypes.c_int, [vp, vp, vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32)]),
     "cpfem_axpby": (ctypes.c_int, [vp, vp, ctypes.c_double, vp, ctypes.c_double, vp]),
     "cpfem_to_csr": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),
@@ -96,6 +97,7 @@ _SIGS = {
     "cpfem_point_counts": (ctypes.c_int, [vp, vp, ctypes.c_int32, vp]),
     "cpfem_krylov_config": (ctypes.c_int, [vp, vp]),
     "cpfem_set_spmv_form": (ctypes.c_int, [vp, ctypes.c_int32]),
+    "cpfem_set_box_columns": (ctypes.c_int, [vp, ctypes.c_int32]),
     "cpfem_set_point_park": (ctypes.c_int, [vp, ctypes.c_int32]),
 }
 

@@ -81,6 +81,11 @@ class NativeContext:
         time, and the rounding of the dot products, never the algorithm."""
         N.check(self._lib.cpfem_set_spmv_form(self.h, int(form)))
 
+    def set_box_columns(self, enable):
+        """Computed node columns of the one-row-per-thread SpMV on a box-numbered mesh on (the
+        default) or off (cpfem_set_box_columns): changes the time, never a result bit."""
+        N.check(self._lib.cpfem_set_box_columns(self.h, 1 if enable else 0))
+
     def krylov_config(self):
         """(SpMV form, SpMV blocks, vector-kernel blocks) of this context's BiCGSTAB."""
         out = (ctypes.c_int32 * 3)()
@@ -99,6 +104,14 @@ class NativeContext:
         col = np.empty((self.n_slots, self.n_nodes), dtype=np.int32)
         N.check(self._lib.cpfem_pattern(self.h, nslot.ctypes.data, col.ctypes.data))
         return nslot, col
+
+    def box_numbering(self):
+        """(nx, ny, nz) nodes if the mesh is a box numbered x-fastest whose nodes off the faces
+        have the 27 stencil columns (cpfem_box_numbering), else (0, 0, 0)."""
+        nslot, col = self.pattern()
+        dims = np.zeros(3, dtype=np.int64)
+        N.check(self._lib.cpfem_box_numbering(self.n_nodes, col.ctypes.data, nslot.ctypes.data, dims.ctypes.data))
+        return tuple(int(d) for d in dims)
 
 
 class BlockEllMatrix:

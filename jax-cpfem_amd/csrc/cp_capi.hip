@@ -157,6 +157,8 @@ struct cpfem_ctx {
   int32_t* d_bflag = nullptr;
   Prof prof;
   int spmv_form = -1;  // BiCGSTAB SpMV form: -1 = picked by size (krylov_spmv_form), else forced
+  int64_t box[3] = {0, 0, 0};  // box numbering (nx, ny, nz) of the pattern, or zeros
+  int box_cols = 1;            // the flat SpMV computes the box numbering's columns (0: loads them)
   double* scratch = nullptr;  // device address of the fixed workspace array on this context's device
 
   int krylov_form() const { return spmv_form >= 0 ? spmv_form : krylov_spmv_form(n_dofs, n_slots); }
@@ -177,6 +179,13 @@ struct cpfem_ctx {
     M.nslot = d_nslot;
     M.n_nodes = n_nodes;
     M.n_slots = n_slots;
+    const bool bx = box_cols && box[0] > 0;
+    M.box_b = bx ? (int32_t)box[0] : 0;
+    M.box_a = bx ? (int32_t)(box[0] * box[1]) : 0;
+    M.box_ny = bx ? (int32_t)box[1] : 0;
+    M.box_nz = bx ? (int32_t)box[2] : 0;
+    M.box_inv_b = bx ? 1.0 / (double)M.box_b : 0.0;
+    M.box_inv_a = bx ? 1.0 / (double)M.box_a : 0.0;
     return M;
   }
   int64_t mat_len() const { return 3LL * n_slots * n_dofs; }
@@ -273,6 +282,35 @@ static int build_geometry(cpfem_ctx* c, const double* pts, const int32_t* cells)
 }
 
 // ------------------------------------------------------------------------- pattern / maps
+// Box numbering of a block-ELL pattern (host): the strides B = nx and A = nx ny are read off
+// the first 27-slot row (its slots 16 and 22 hold the neighbours at +B and +A), then every
+// node off the box's faces must have exactly the 27 columns i + dz A + dy B + dx, ascending.
+// dims = (nx, ny, nz), or zeros if the pattern is not so numbered.
+static void box_numbering(int64_t nn, const int32_t* col, const int32_t* nslot, int64_t dims[3]) {
+  dims[0] = dims[1] = dims[2] = 0;
+  int ms = 0;
+  for (int64_t i = 0; i < nn; ++i) ms = std::max(ms, (int)nslot[i]);
+  if (ms != 27) return;
+  int64_t i0 = 0;
+  while (i0 < nn && nslot[i0] != 27) ++i0;
+  if (i0 == nn) return;
+  const int64_t B = (int64_t)col[16 * nn + i0] - i0, A = (int64_t)col[22 * nn + i0] - i0;
+  if (B < 3 || A < 3 * B || A % B != 0 || nn % A != 0) return;
+  const int64_t nx = B, ny = A / B, nz = nn / A;
+  if (nz < 3) return;
+  for (int64_t iz = 1; iz + 1 < nz; ++iz)
+    for (int64_t iy = 1; iy + 1 < ny; ++iy)
+      for (int64_t ix = 1; ix + 1 < nx; ++ix) {
+        const int64_t i = ix + B * iy + A * iz;
+        if (nslot[i] != 27) return;
+        for (int s = 0; s < 27; ++s)
+          if ((int64_t)col[(int64_t)s * nn + i] != i + (s / 9 - 1) * A + (s / 3 % 3 - 1) * B + (s % 3 - 1)) return;
+      }
+  dims[0] = nx;
+  dims[1] = ny;
+  dims[2] = nz;
+}
+
 static int build_maps(cpfem_ctx* c, const int32_t* cells) {
   const int64_t nn = c->n_nodes, nc = c->n_cells;
   // node -> (cell, local) list, ascending cell
@@ -309,6 +347,7 @@ static int build_maps(cpfem_ctx* c, const int32_t* cells) {
     for (int s = 0; s < maxs; ++s) c->h_col[(size_t)s * nn + i] = s < (int)v.size() ? v[s] : (int32_t)i;
     c->h_dslot[i] = (int32_t)(std::lower_bound(v.begin(), v.end(), (int32_t)i) - v.begin());
   }
+  box_numbering(nn, c->h_col.data(), c->h_nslot.data(), c->box);
   // transpose map: slot s of row j points at node i; tslot gives the slot of node j in row i
   std::vector<int32_t> tslot((size_t)maxs * nn, 0);
   for (int64_t j = 0; j < nn; ++j)
@@ -890,6 +929,12 @@ int cpfem_band_order(int64_t n_nodes, const int32_t* col, const int32_t* nslot, 
   return band_ordering(n_nodes, col, nslot, ip, bandwidth);
 }
 
+int cpfem_box_numbering(int64_t n_nodes, const int32_t* col, const int32_t* nslot, int64_t* dims) {
+  if (n_nodes <= 0 || !col || !nslot || !dims) return fail(CPFEM_EBADARG, "null argument");
+  box_numbering(n_nodes, col, nslot, dims);
+  return CPFEM_OK;
+}
+
 // Band storage is capped (BAND_MAX_BYTES); larger systems are refused with a message that
 // names the Krylov solver.
 static constexpr double BAND_MAX_BYTES = 64.0 * (1ull << 30);
@@ -1005,6 +1050,13 @@ int cpfem_set_spmv_form(cpfem_ctx* c, int32_t form) {
     return fail(CPFEM_EBADARG, form == 3 ? "the one-row-per-thread SpMV form needs a 27-slot (hex8 box-like) mesh"
                                          : "the all-slots SpMV form needs fewer than 2^28 rows");
   c->spmv_form = form;
+  return CPFEM_OK;
+}
+
+int cpfem_set_box_columns(cpfem_ctx* c, int32_t enable) {
+  if (!c) return fail(CPFEM_EBADARG, "null context");
+  if (enable != 0 && enable != 1) return fail(CPFEM_EBADARG, "enable must be 0 or 1");
+  c->box_cols = enable;
   return CPFEM_OK;
 }
 

@@ -55,6 +55,12 @@ struct BellView {
   const int32_t* nslot;  // [n_nodes]
   int64_t n_nodes;
   int n_slots;
+  // Box numbering (box_b > 0): the nodes of a box of box_b x (box_a / box_b) x box_nz nodes are
+  // numbered i = ix + box_b iy + box_a iz, and every node off the box's faces has the 27
+  // columns i + dz box_a + dy box_b + dx (dx, dy, dz in {-1, 0, 1}), in that (ascending)
+  // order, which the one-row-per-thread SpMV computes instead of loading.
+  int32_t box_b, box_a, box_ny, box_nz;
+  double box_inv_b, box_inv_a;  // 1 / box_b, 1 / box_a
 };
 
 // Parking of the assembly point kernel's slow lanes.  The lanes of a wave iterate their local

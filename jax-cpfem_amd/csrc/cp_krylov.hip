@@ -762,8 +762,26 @@ k_kst_all(BellView M, const TM* __restrict__ vals, KVec v, KrylovWork w, int par
 // release fence per block, an L2 write-back on gfx950: 286 us per SpMV at 64^3.)
 constexpr int FBS = 256;
 
+// Node columns of row r.  With a box numbering (M.box_b > 0) a node off the box's faces
+// computes them (i + dz A + dy B + dx, the ascending order its stored slots have), so only
+// face nodes load theirs: 27 x 4 bytes per node and the loads' round trip before the operand
+// gathers saved.  The node's (ix, iy, iz) come from two floor divisions in FP64: (i + 0.5) / A
+// lies at least 0.5 / A from an integer, far beyond the product's rounding for i < 2^30.
 __device__ __forceinline__ void flat_cols(const BellView& M, int64_t r, int32_t (&j)[27]) {
   const int64_t i = r / 3;
+  if (M.box_b > 0) {  // uniform over the grid
+    const int32_t ii = (int32_t)i;
+    const int32_t iz = (int32_t)(((double)ii + 0.5) * M.box_inv_a);
+    const int32_t rz = ii - iz * M.box_a;
+    const int32_t iy = (int32_t)(((double)rz + 0.5) * M.box_inv_b);
+    const int32_t ix = rz - iy * M.box_b;
+    if ((uint32_t)(ix - 1) < (uint32_t)(M.box_b - 2) && (uint32_t)(iy - 1) < (uint32_t)(M.box_ny - 2) &&
+        (uint32_t)(iz - 1) < (uint32_t)(M.box_nz - 2)) {
+#pragma unroll
+      for (int s = 0; s < 27; ++s) j[s] = ii + (s / 9 - 1) * M.box_a + (s / 3 % 3 - 1) * M.box_b + (s % 3 - 1);
+      return;
+    }
+  }
 #pragma unroll
   for (int s = 0; s < 27; ++s) j[s] = M.col[(int64_t)s * M.n_nodes + i];
 }

@@ -9,6 +9,23 @@ from conftest import GOLDEN
 FCC_304 = dict(xm=0.02, gss_a=2.0, h=500.0, t_sat=300.0, r=1.4, C11=204.6e3, C12=137.7e3, C44=126.2e3)
 
 
+def block_ell_pattern(cells, nn):
+    """Block-ELL node pattern of a hex mesh as the library stores it: [slots][nn] node columns,
+    ascending, padded with the row's own node, and the slot counts."""
+    nb = [set() for _ in range(nn)]
+    for c in cells:
+        for a in c:
+            nb[a].update(int(b) for b in c)
+    ms = max(len(s) for s in nb)
+    col = np.empty((ms, nn), dtype=np.int32)
+    nslot = np.empty(nn, dtype=np.int32)
+    for i, s in enumerate(nb):
+        v = sorted(s)
+        nslot[i] = len(v)
+        col[:, i] = v + [i] * (ms - len(v))
+    return col, nslot
+
+
 def slip_table(name):
     return np.loadtxt(os.path.join(GOLDEN, name))
 

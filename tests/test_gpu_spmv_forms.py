@@ -115,7 +115,7 @@ def test_all_slots_form_reproduces_one_slot_form_bitwise():
 
 @pytest.mark.parametrize("n", [4, 16, 28])
 def test_flat_form_solves_first_newton_system(n):
-    """The one-row-per-thread form (3), forced: one partial sum per 512-row block, added by the
+    """The one-row-per-thread form (3), forced: one partial sum per 256-row block, added by the
     consuming kernel, so its dot products run in a different order from the tile forms'.
     FP64 and mixed solutions meet the same true-residual bar as the size-picked form and land
     within 1e-6 of it.  Iteration counts are not compared: BiCGSTAB's count moves with the
@@ -144,3 +144,37 @@ def test_flat_form_solves_first_newton_system(n):
         assert res <= bar, (prec, res, bar)
         assert err < 1e-6, (prec, err)
     N.check(prob.ctx._lib.cpfem_set_spmv_form(prob.ctx.h, -1))
+
+
+@pytest.mark.parametrize("n", [4, 16, 28])
+def test_flat_form_box_columns_bitwise(n):
+    """On a box mesh's pattern the one-row-per-thread form computes the columns of the nodes
+    off the box's faces (cpfem_box_numbering finds the numbering) instead of loading them; the
+    columns are the same, so with computed columns on and off the solutions and iteration
+    counts are bitwise equal, FP64 and mixed."""
+    import ctypes
+    from cpfem_amd import _native as N
+    from cpfem_amd.solver import krylov_solve
+    prob, mat, b, x0 = _first_system(n)
+    L, h = prob.ctx._lib, prob.ctx.h
+    nn = prob.ctx.n_dofs // 3
+    nslot = np.empty(nn, dtype=np.int32)
+    col = np.empty((27, nn), dtype=np.int32)
+    N.check(L.cpfem_pattern(h, nslot.ctypes.data, col.ctypes.data))
+    dims = np.zeros(3, dtype=np.int64)
+    N.check(L.cpfem_box_numbering(nn, col.ctypes.data, nslot.ctypes.data, dims.ctypes.data))
+    assert list(dims) == [n + 1] * 3
+    N.check(L.cpfem_set_spmv_form(h, 3))
+    out = {}
+    for on in (1, 0):
+        N.check(L.cpfem_set_box_columns(h, on))
+        for prec in ("fp64", "mixed"):
+            x = krylov_solve(prob, mat, b, x0, dict(precision=prec))
+            out[on, prec] = (x.cpu().numpy(), prob.solver_info[-1]["iterations"])
+    N.check(L.cpfem_set_box_columns(h, 1))
+    N.check(L.cpfem_set_spmv_form(h, -1))
+    for prec in ("fp64", "mixed"):
+        print(f"n={n} {prec}: {out[1, prec][1]} iterations with computed columns, {out[0, prec][1]} with loaded",
+              flush=True)
+        np.testing.assert_array_equal(out[1, prec][0], out[0, prec][0])
+        assert out[1, prec][1] == out[0, prec][1]

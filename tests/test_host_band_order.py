@@ -8,6 +8,8 @@ import os
 import numpy as np
 import pytest
 
+from helpers import block_ell_pattern as _pattern
+
 pytest.importorskip("torch")
 
 
@@ -16,22 +18,6 @@ def _lib():
     if not os.path.exists(N.LIB_PATH):
         pytest.skip("libcpfem_hip.so not built")
     return N.load_library()
-
-
-def _pattern(cells, nn):
-    """Block-ELL node pattern ([slots][nn] with self-padding, slot counts) of a hex mesh."""
-    nb = [set() for _ in range(nn)]
-    for c in cells:
-        for a in c:
-            nb[a].update(int(b) for b in c)
-    ms = max(len(s) for s in nb)
-    col = np.empty((ms, nn), dtype=np.int32)
-    nslot = np.empty(nn, dtype=np.int32)
-    for i, s in enumerate(nb):
-        v = sorted(s)
-        nslot[i] = len(v)
-        col[:, i] = v + [i] * (ms - len(v))
-    return col, nslot
 
 
 def _order(L, col, nslot):
