@@ -47,13 +47,16 @@ def log(msg):
 
 
 # ------------------------------------------------------------------------------ byte / flop models
-def spmv_bytes(prob, value_bytes=8, computed_columns=False):
-    """Algorithmic HBM bytes of one Krylov SpMV launch (k_kpq / k_kst, equal counts): the
-    block values (value_bytes each), one int32 column index per 3x3 block (plus the slot
-    counts, read unless every row is padded to 27 slots; padding is not counted), the two FP64
-    vectors the operand is formed from (read once each), and five FP64 row vectors (k_kpq:
-    reads rhat and the Jacobi vector, writes phat, q, M q; k_kst: reads r, q, writes s, shat,
-    t).  computed_columns: the one-row-per-thread form on a box-numbered mesh computes the
+def spmv_bytes(prob, value_bytes=8, computed_columns=False, one_operand=False):
+    """Algorithmic HBM bytes of one Krylov SpMV launch (mean of the two kernels): the block
+    values (value_bytes each), one int32 column index per 3x3 block (plus the slot counts, read
+    unless every row is padded to 27 slots; padding is not counted) and FP64 row vectors, each
+    read or written once.  Pair recurrence (k_kpq / k_kst, 7 vectors each): the two vectors
+    the operand is formed from, and k_kpq reads rhat and the Jacobi vector and writes phat, q,
+    M q; k_kst reads r, q and writes s, shat, t.  One-operand recurrence (k_kpq_flat1: 7,
+    k_kst_flat1: 6): k_kpq_flat1 gathers phat, reads rhat, the Jacobi vector and rh and writes
+    q and the (rh, qh) pair; k_kst_flat1 gathers the pair, reads r, q, rhat and writes t.
+    computed_columns: the one-row-per-thread form on a box-numbered mesh computes the
     columns of the nodes off the box's faces, so only the face nodes' indices count."""
     nslot, _ = prob.ctx.pattern()
     nnzb = int(nslot.sum())
@@ -63,7 +66,8 @@ def spmv_bytes(prob, value_bytes=8, computed_columns=False):
         nx, ny, nz = prob.ctx.box_numbering()
         if nx > 0:
             idx -= 4 * 27 * (nx - 2) * (ny - 2) * (nz - 2)
-    return 9 * value_bytes * nnzb + idx + 7 * 8 * n, nnzb
+    vec = (7 + 6) / 2 if one_operand else 7
+    return round(9 * value_bytes * nnzb + idx + vec * 8 * n), nnzb
 
 
 def point_flops(n_slip, tangent_points, newton_iters, residual_evals):
@@ -445,7 +449,8 @@ def main():
     value = Q * K / T
     vb = 4 if args.precision == "mixed" else 8
     form = prob.ctx.krylov_config()[0]
-    sb, nnzb = spmv_bytes(prob, value_bytes=vb, computed_columns=form == 3 and not args.no_box_columns)
+    sb, nnzb = spmv_bytes(prob, value_bytes=vb, computed_columns=form == 3 and not args.no_box_columns,
+                          one_operand=form == 3 and vb == 8)
     cus = torch.cuda.get_device_properties(0).multi_processor_count
     tm = "float" if vb == 4 else "double"
     names = spmv_kernel_names(form, tm)
@@ -536,6 +541,10 @@ def main():
     if not args.no_adjoint:
         if args.config == 5 and S == 0:
             log("[bench] load-step adjoint (900 Euler angles) ...")
+            # one untimed gradient first: the first backward of a process pays one-time costs
+            # (first launches of the VJP and transpose kernels, the adjoint's device buffers),
+            # 0.2 s of scatter in an otherwise ~0.5 s backward
+            step_adjoint(prob, w, S, szz[0], "fp64")
             a64 = step_adjoint(prob, w, S, szz[0], "fp64")
             amx = step_adjoint(prob, w, S, szz[0], "mixed")
             g64, gmx = a64.pop("grad"), amx.pop("grad")
@@ -544,7 +553,7 @@ def main():
                                 loss_rel_err=abs(amx["loss"] - a64["loss"]) / abs(a64["loss"]),
                                 grad_max_rel_err=float(np.abs(gmx - g64).max() / np.abs(g64).max()))
             a64.update(n_angles=int(g64.size), grad_norm=float(np.linalg.norm(g64)), szz_rel_err_vs_timed_step=abs(
-                a64["szz"] - szz[0]) / abs(szz[0]))
+                a64["szz"] - szz[0]) / abs(szz[0]), warmup_gradients=1)
             detail["adjoint_load_step"] = a64
             log(f"[bench] load-step adjoint: {json.dumps(a64)}")
         if not args.no_history_adjoint:

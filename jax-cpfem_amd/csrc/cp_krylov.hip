@@ -1011,7 +1011,9 @@ __global__ void __launch_bounds__(FBS) k_kpq_flat1(BellView M, const double* __r
   flat_clear_unowned<1>({PS_RQ}, w.partial, w.ps);
 }
 
-// k_kst_flat with the partials of <rhat,s> and <rhat,t> as well.
+// k_kst_flat with the partials of <rhat,s> and <rhat,t> as well.  It stores only t: the x/r
+// update recomputes s = r - alpha q and shat = rh - alpha qh (same operations, same bits) from
+// r, q and the (rh, qh) pair it reads anyway, two vector writes fewer per iteration.
 __global__ void __launch_bounds__(FBS) k_kst_flat1(BellView M, const double* __restrict__ vals, KVec v, KrylovWork w,
                                                    int par) {
   __shared__ double sh[5 * (FBS / 64)];
@@ -1041,9 +1043,7 @@ __global__ void __launch_bounds__(FBS) k_kst_flat1(BellView M, const double* __r
   double d[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   if (r < n) {
     const double tv = __fma_rn(-alpha, a2, a1);
-    const double s = e0 - alpha * e1;
-    v.s[r] = s;
-    v.shat[r] = __fma_rn(-alpha, e2.y, e2.x);
+    const double s = __fma_rn(-alpha, e1, e0);  // k_kxr1 forms s and shat again from r, q, rh, qh
     v.t[r] = tv;
     d[0] = s * s;
     d[1] = tv * s;
@@ -1062,17 +1062,18 @@ __global__ void __launch_bounds__(FBS) k_kst_flat1(BellView M, const double* __r
   flat_clear_unowned<5>({PS_SS, PS_TS, PS_TT, PS_RS, PS_RT}, w.partial, w.ps);
 }
 
-// k_kxr for the one-operand recurrence: also forms the next phat.
+// k_kxr for the one-operand recurrence: also forms the next phat, and s and shat itself.
 __global__ void __launch_bounds__(KBS) k_kxr1(KVec v, KrylovWork w, int par) {
   __shared__ double sh[5 * (KBS / 64)];
   KrylovScalars* sc = w.sc;
   const int done = sc->done;
   const double alpha = sc->alpha2[par], rho = sc->rho2[par], atol2 = sc->atol2;
   const int64_t i0 = (int64_t)blockIdx.x * KBS + threadIdx.x;
-  double x = 0, ph = 0, shv = 0, s = 0, t = 0, qh = 0, rhat = 0, invd = 0;
+  double x = 0, ph = 0, r0 = 0, q = 0, t = 0, rhat = 0, invd = 0;
+  double2 rq = make_double2(0.0, 0.0);
   if (i0 < w.n) {  // issued before the done test (one round trip for both)
-    x = v.x[i0]; ph = v.phat[i0]; shv = v.shat[i0]; s = v.s[i0]; t = v.t[i0];
-    qh = v.rq[i0].y; rhat = v.rhat[i0]; invd = v.invd[i0];
+    x = v.x[i0]; ph = v.phat[i0]; r0 = v.r[i0]; q = v.q[i0]; t = v.t[i0];
+    rq = v.rq[i0]; rhat = v.rhat[i0]; invd = v.invd[i0];
   }
   if (done) return;
   double sums[5];
@@ -1086,9 +1087,10 @@ __global__ void __launch_bounds__(KBS) k_kxr1(KVec v, KrylovWork w, int par) {
   double rr = 0.0, rhr = 0.0;
   for (int64_t i = i0; i < w.n; i += (int64_t)w.nb * KBS) {
     if (i != i0) {
-      x = v.x[i]; ph = v.phat[i]; shv = v.shat[i]; s = v.s[i]; t = v.t[i];
-      qh = v.rq[i].y; rhat = v.rhat[i]; invd = v.invd[i];
+      x = v.x[i]; ph = v.phat[i]; r0 = v.r[i]; q = v.q[i]; t = v.t[i];
+      rq = v.rq[i]; rhat = v.rhat[i]; invd = v.invd[i];
     }
+    const double s = __fma_rn(-alpha, q, r0), shv = __fma_rn(-alpha, rq.y, rq.x), qh = rq.y;
     double r;
     if (early) {
       v.x[i] = x + alpha * ph;
