@@ -151,7 +151,7 @@ int cpfem_solve(cpfem_ctx* ctx, void* stream, const double* mat, const double* b
 /* cpfem_solve in FP64 without any device allocation.  No counterpart in the reference: it is
    for callers that cannot allocate device memory (code under HIP graph capture, tight serving
    loops, harnesses that forbid hipMalloc).  The Krylov workspace (scalars, partial sums,
-   13 n_dofs doubles of vectors) lives in a fixed device array of the library, about 20 MiB,
+   11 n_dofs doubles of vectors) lives in a fixed device array of the library, about 17 MiB,
    which each device that the library runs on holds until the process ends; the array's
    device address is looked up when a context is created on that device.  Limits: n_dofs <=
    CPFEM_NOALLOC_MAX_DOFS (larger systems return CPFEM_EBADARG and must use cpfem_solve);

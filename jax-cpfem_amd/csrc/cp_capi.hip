@@ -142,7 +142,7 @@ struct cpfem_ctx {
   // Krylov
   KrylovScalars* d_ksc = nullptr;
   double* d_kpartial = nullptr;  // [PS_COUNT][krylov blocks]
-  double* kv64[12] = {};  // r rhat (spare) q phat s shat t invd (spare) rp rq; rp, rq hold 2 n
+  double* kv64[12] = {};  // r rhat (spare) q phat (spare) (spare) t invd (spare) rp rq; rp, rq hold 2 n
   float* d_mat32 = nullptr;
   double *d_r64 = nullptr, *d_tmp64 = nullptr;
   // pinned host staging
@@ -493,8 +493,6 @@ static KVec kvec(double** v, double* x) {
 
   k.q = v[3];
   k.phat = v[4];
-  k.s = v[5];
-  k.shat = v[6];
   k.t = v[7];
   k.invd = v[8];
   k.rp = reinterpret_cast<double2*>(v[10]);
@@ -505,12 +503,12 @@ static KVec kvec(double** v, double* x) {
 // Fixed workspace of cpfem_solve_noalloc: a file-scope device array (each device holds its own
 // copy until the process ends).  Host code never passes the array's name to a kernel: its
 // device address is looked up once per device (scratch_address) and used from then on.
-// Layout: Krylov scalars, partial sums [PS_SLOTS][NOALLOC_PARTIALS], then 13 n doubles of
+// Layout: Krylov scalars, partial sums [PS_SLOTS][NOALLOC_PARTIALS], then 11 n doubles of
 // vectors for n <= CPFEM_NOALLOC_MAX_DOFS.
 constexpr int NOALLOC_SCALARS = 32;  // doubles reserved for KrylovScalars (256 B)
-constexpr int PS_SLOTS = 8;          // partial-sum slots (PS_COUNT = 7, rounded up)
+constexpr int PS_SLOTS = 8;          // partial-sum slots (PS_COUNT)
 constexpr int NOALLOC_PARTIALS = 2048;
-constexpr size_t NOALLOC_DOUBLES = (size_t)NOALLOC_SCALARS + (size_t)PS_SLOTS * NOALLOC_PARTIALS + 13ull * CPFEM_NOALLOC_MAX_DOFS;
+constexpr size_t NOALLOC_DOUBLES = (size_t)NOALLOC_SCALARS + (size_t)PS_SLOTS * NOALLOC_PARTIALS + 11ull * CPFEM_NOALLOC_MAX_DOFS;
 static_assert(sizeof(KrylovScalars) <= NOALLOC_SCALARS * sizeof(double), "Krylov scalars exceed their slot");
 static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
 
@@ -1110,8 +1108,8 @@ int cpfem_solve_noalloc(cpfem_ctx* c, void* stream, const double* mat, const dou
   kb.partial = p;
   p += (size_t)PS_SLOTS * NOALLOC_PARTIALS;
   double* vec0 = p;
-  // the vectors this solve uses (kv64 slots 2 and 9 are spare); rp and rq hold 2 n each
-  for (int i : {0, 1, 3, 4, 5, 6, 7, 8}) { kb.v[i] = p; p += n; }
+  // the vectors this solve uses (kv64 slots 2, 5, 6 and 9 are spare); rp and rq hold 2 n each
+  for (int i : {0, 1, 3, 4, 7, 8}) { kb.v[i] = p; p += n; }
   kb.v[10] = p; p += 2 * n;
   kb.v[11] = p; p += 2 * n;
   kb.tmp64 = p; p += n;

@@ -197,15 +197,16 @@ struct KrylovWork {
   KrylovScalars* sc;
 };
 
-// BiCGSTAB vectors.  Besides JAX's x, r, rhat, q, phat = M p, s, shat = M s, t the
-// iteration keeps rh = M r, qh = M q and pp = M (p - omega q), so that the next phat =
-// rh + beta pp and shat = rh - alpha qh can be formed inside the SpMV kernels.  They are
+// BiCGSTAB vectors.  Besides JAX's x, r, rhat, q, phat = M p, t the iteration keeps
+// rh = M r, qh = M q and pp = M (p - omega q), so that the next phat = rh + beta pp and
+// shat = rh - alpha qh can be formed inside the SpMV kernels; s = r - alpha q and shat are
+// not stored (the x/r update forms them again from r, q and the (rh, qh) pair).  They are
 // stored as interleaved pairs rp = (rh, pp) and rq = (rh, qh), so that each SpMV gathers its
 // two operand vectors with one 16-byte load per column entry.  rh is written into rp (x/r
 // update, init, residual replacement); the first SpMV kernel writes rq whole, with its own
 // row's rh, when it forms qh.
 struct KVec {
-  double *x, *r, *rhat, *q, *phat, *s, *shat, *t;
+  double *x, *r, *rhat, *q, *phat, *t;
   double2 *rp, *rq;
   const double* invd;
 };

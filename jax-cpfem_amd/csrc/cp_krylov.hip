@@ -667,8 +667,8 @@ __device__ __forceinline__ void kpq_body(const BellView& M, const TM* __restrict
   put_partials<BS, 1>({d}, {PS_RQ}, w.partial, w.ps, sh, run);
 }
 
-// t = A shat with shat = rh - alpha qh, alpha = rho / <rhat,q>.  Stores s = r - alpha q, shat,
-// t and the partials of <t,s>, <t,t>, <s,s>.
+// t = A shat with shat = rh - alpha qh, alpha = rho / <rhat,q>.  Stores t and the partials
+// of <t,s>, <t,t>, <s,s> (s = r - alpha q); the x/r update forms s and shat again.
 template <typename TM, int G, bool ALL>
 __device__ __forceinline__ void kst_body(const BellView& M, const TM* __restrict__ vals, const KVec& v, const KrylovWork& w,
                                          int par) {
@@ -686,12 +686,10 @@ __device__ __forceinline__ void kst_body(const BellView& M, const TM* __restrict
   const TileRun tiles(run, gridDim.x, n, BR);
   int64_t tile = tiles.first;
   int64_t r = tile * BR + t.group * SP_ROWS + t.lane;
-  double e0 = 0.0, e1 = 0.0;             // r, q of the row
-  double2 e2 = make_double2(0.0, 0.0);  // (rh, qh) of the row
+  double e0 = 0.0, e1 = 0.0;  // r, q of the row
   if (t.part == 0 && r < n) {
     e0 = v.r[r];
     e1 = v.q[r];
-    e2 = rq[r];
   }
   double a1, a2;
   double p[1] = {0.0};
@@ -706,9 +704,7 @@ __device__ __forceinline__ void kst_body(const BellView& M, const TM* __restrict
   for (;;) {
     if (t.part == 0 && r < n) {
       const double tv = __fma_rn(-alpha, a2, a1);
-      const double s = e0 - alpha * e1;
-      v.s[r] = s;
-      v.shat[r] = __fma_rn(-alpha, e2.y, e2.x);
+      const double s = __fma_rn(-alpha, e1, e0);  // k_kxr forms s and shat again
       v.t[r] = tv;
       ts += tv * s;
       tt += tv * tv;
@@ -719,7 +715,6 @@ __device__ __forceinline__ void kst_body(const BellView& M, const TM* __restrict
     if (t.part == 0 && r < n) {
       e0 = v.r[r];
       e1 = v.q[r];
-      e2 = rq[r];
     }
     const SlotCols jc = jn;
     load_next_cols<ALL>(jn, M, r + BR, n, tile + 1 < tiles.end, t.part);
@@ -923,11 +918,9 @@ __global__ void __launch_bounds__(FBS) k_kst_flat(BellView M, const TM* __restri
 #pragma unroll 4
   for (int i = threadIdx.x; i < (int)gridDim.x; i += FBS) p[0] += w.partial[PS_RQ * w.ps + i];
   double a1 = 0.0, a2 = 0.0, e0 = 0.0, e1 = 0.0;
-  double2 e2 = make_double2(0.0, 0.0);
   if (r < n) {
     e0 = v.r[r];
     e1 = v.q[r];
-    e2 = v.rq[r];
     spmv2_flat<TM>(M, vals, v.rq, r, j, a1, a2);
   }
   flat_block_sums<1>(p, sh);
@@ -936,9 +929,7 @@ __global__ void __launch_bounds__(FBS) k_kst_flat(BellView M, const TM* __restri
   double d[3] = {0.0, 0.0, 0.0};
   if (r < n) {
     const double tv = __fma_rn(-alpha, a2, a1);
-    const double s = e0 - alpha * e1;
-    v.s[r] = s;
-    v.shat[r] = __fma_rn(-alpha, e2.y, e2.x);
+    const double s = __fma_rn(-alpha, e1, e0);  // k_kxr forms s and shat again
     v.t[r] = tv;
     d[0] = s * s;
     d[1] = tv * s;
@@ -1029,11 +1020,9 @@ __global__ void __launch_bounds__(FBS) k_kst_flat1(BellView M, const double* __r
 #pragma unroll 4
   for (int i = threadIdx.x; i < (int)gridDim.x; i += FBS) p[0] += w.partial[PS_RQ * w.ps + i];
   double a1 = 0.0, a2 = 0.0, e0 = 0.0, e1 = 0.0, e3 = 0.0;
-  double2 e2 = make_double2(0.0, 0.0);
   if (r < n) {
     e0 = v.r[r];
     e1 = v.q[r];
-    e2 = v.rq[r];
     e3 = v.rhat[r];
     spmv2_flat<double>(M, vals, v.rq, r, j, a1, a2);
   }
@@ -1110,17 +1099,20 @@ __global__ void __launch_bounds__(KBS) k_kxr1(KVec v, KrylovWork w, int par) {
 }
 
 // omega, x and r updates; stores rh = M r, pp = phat - omega qh and the partials of <r,r>,
-// <rhat,r>.  Scalars, the first row's operands and the partials are loaded together.
+// <rhat,r>.  s = r - alpha q and shat = rh - alpha qh are formed here again from r, q and the
+// (rh, qh) pair (the second SpMV kernel stores only t; the same operations give the same
+// bits).  Scalars, the first row's operands and the partials are loaded together.
 __global__ void __launch_bounds__(KBS) k_kxr(KVec v, KrylovWork w, int par) {
   __shared__ double sh[3 * (KBS / 64)];
   KrylovScalars* sc = w.sc;
   const int done = sc->done;
   const double alpha = sc->alpha2[par], atol2 = sc->atol2;
   const int64_t i0 = (int64_t)blockIdx.x * KBS + threadIdx.x;
-  double x = 0, ph = 0, shv = 0, s = 0, t = 0, qh = 0, rhat = 0, invd = 0;
+  double x = 0, ph = 0, r0 = 0, q = 0, t = 0, rhat = 0, invd = 0;
+  double2 rq = make_double2(0.0, 0.0);
   if (i0 < w.n) {  // issued before the done test (one round trip for both)
-    x = v.x[i0]; ph = v.phat[i0]; shv = v.shat[i0]; s = v.s[i0]; t = v.t[i0];
-    qh = v.rq[i0].y; rhat = v.rhat[i0]; invd = v.invd[i0];
+    x = v.x[i0]; ph = v.phat[i0]; r0 = v.r[i0]; q = v.q[i0]; t = v.t[i0];
+    rq = v.rq[i0]; rhat = v.rhat[i0]; invd = v.invd[i0];
   }
   if (done) return;
   double sums[3];
@@ -1132,9 +1124,10 @@ __global__ void __launch_bounds__(KBS) k_kxr(KVec v, KrylovWork w, int par) {
   double rr = 0.0, rhr = 0.0;
   for (int64_t i = i0; i < w.n; i += (int64_t)w.nb * KBS) {
     if (i != i0) {
-      x = v.x[i]; ph = v.phat[i]; shv = v.shat[i]; s = v.s[i]; t = v.t[i];
-      qh = v.rq[i].y; rhat = v.rhat[i]; invd = v.invd[i];
+      x = v.x[i]; ph = v.phat[i]; r0 = v.r[i]; q = v.q[i]; t = v.t[i];
+      rq = v.rq[i]; rhat = v.rhat[i]; invd = v.invd[i];
     }
+    const double s = __fma_rn(-alpha, q, r0), shv = __fma_rn(-alpha, rq.y, rq.x), qh = rq.y;
     double r;
     if (early) {
       v.x[i] = x + alpha * ph;
