@@ -51,11 +51,11 @@ def spmv_bytes(prob, value_bytes=8, computed_columns=False, one_operand=False):
     """Algorithmic HBM bytes of one Krylov SpMV launch (mean of the two kernels): the block
     values (value_bytes each), one int32 column index per 3x3 block (plus the slot counts, read
     unless every row is padded to 27 slots; padding is not counted) and FP64 row vectors, each
-    read or written once.  Pair recurrence (k_kpq / k_kst, 7 vectors each): the two vectors
-    the operand is formed from, and k_kpq reads rhat and the Jacobi vector and writes phat, q,
-    M q; k_kst reads r, q and writes s, shat, t.  One-operand recurrence (k_kpq_flat1: 7,
-    k_kst_flat1: 6): k_kpq_flat1 gathers phat, reads rhat, the Jacobi vector and rh and writes
-    q and the (rh, qh) pair; k_kst_flat1 gathers the pair, reads r, q, rhat and writes t.
+    read or written once.  Pair recurrence (k_kpq: 7 vectors, k_kst: 5): the two vectors the
+    operand is formed from, and k_kpq reads rhat and the Jacobi vector and writes phat, q and
+    M q; k_kst reads r, q and writes t.  One-operand recurrence (k_kpq_flat1: 7, k_kst_flat1:
+    6): k_kpq_flat1 gathers phat, reads rhat, the Jacobi vector and rh and writes q and the
+    (rh, qh) pair; k_kst_flat1 gathers the pair, reads r, q, rhat and writes t.
     computed_columns: the one-row-per-thread form on a box-numbered mesh computes the
     columns of the nodes off the box's faces, so only the face nodes' indices count."""
     nslot, _ = prob.ctx.pattern()
@@ -66,7 +66,7 @@ def spmv_bytes(prob, value_bytes=8, computed_columns=False, one_operand=False):
         nx, ny, nz = prob.ctx.box_numbering()
         if nx > 0:
             idx -= 4 * 27 * (nx - 2) * (ny - 2) * (nz - 2)
-    vec = (7 + 6) / 2 if one_operand else 7
+    vec = (7 + 6) / 2 if one_operand else (7 + 5) / 2
     return round(9 * value_bytes * nnzb + idx + vec * 8 * n), nnzb
 
 
