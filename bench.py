@@ -85,11 +85,13 @@ def point_flops(n_slip, tangent_points, newton_iters, residual_evals):
 
 def spmv_kernel_names(form, tm):
     """Names of the BiCGSTAB SpMV kernel pair of an SpMV form (as the library reports it,
-    cpfem_krylov_config): 3 = one row per thread, 2 = 128-row tiles, 1 = 64-row tiles, 0 =
-    64-row tiles with every slot's loads in flight."""
-    if form == 3 and tm == "double":  # the one-operand recurrence of the FP64 flat form
+    cpfem_krylov_config): 4 = one row per thread over XCD-local row ranges, 3 = one row per
+    thread, 2 = 128-row tiles, 1 = 64-row tiles, 0 = 64-row tiles with every slot's loads in
+    flight."""
+    if form in (3, 4) and tm == "double":  # the one-operand recurrence of the FP64 flat forms
         return ["cpfem::k_kpq_flat1", "cpfem::k_kst_flat1"]
-    return {3: [f"cpfem::k_kpq_flat<{tm}>", f"cpfem::k_kst_flat<{tm}>"],
+    return {4: [f"cpfem::k_kpq_flat<{tm}>", f"cpfem::k_kst_flat<{tm}>"],
+            3: [f"cpfem::k_kpq_flat<{tm}>", f"cpfem::k_kst_flat<{tm}>"],
             2: [f"cpfem::k_kpq<{tm}, 2>", f"cpfem::k_kst<{tm}, 2>"],
             1: [f"cpfem::k_kpq<{tm}, 1>", f"cpfem::k_kst<{tm}, 1>"],
             0: [f"cpfem::k_kpq_all<{tm}>", f"cpfem::k_kst_all<{tm}>"]}[form]
@@ -368,7 +370,7 @@ def main():
                     help="park the point kernel's lanes when at most this many of a wave still iterate "
                          "(0: never; -1: the library's default)")
     ap.add_argument("--spmv-form", type=int, default=-1,
-                    help="force the BiCGSTAB SpMV form (0-3; -1: the library's choice by size)")
+                    help="force the BiCGSTAB SpMV form (0-4; -1: the library's choice by size)")
     ap.add_argument("--no-box-columns", action="store_true",
                     help="load every column in the one-row-per-thread SpMV (default: computed on box meshes)")
     ap.add_argument("--history-adjoint-mesh", type=int, default=16)
@@ -449,8 +451,8 @@ def main():
     value = Q * K / T
     vb = 4 if args.precision == "mixed" else 8
     form = prob.ctx.krylov_config()[0]
-    sb, nnzb = spmv_bytes(prob, value_bytes=vb, computed_columns=form == 3 and not args.no_box_columns,
-                          one_operand=form == 3 and vb == 8)
+    sb, nnzb = spmv_bytes(prob, value_bytes=vb, computed_columns=form >= 3 and not args.no_box_columns,
+                          one_operand=form >= 3 and vb == 8)
     cus = torch.cuda.get_device_properties(0).multi_processor_count
     tm = "float" if vb == 4 else "double"
     names = spmv_kernel_names(form, tm)

@@ -202,14 +202,15 @@ int cpfem_spmv(cpfem_ctx* ctx, void* stream, const double* mat, const double* x,
 int cpfem_norm(cpfem_ctx* ctx, void* stream, const double* v, double* out, int32_t* point_status);
 
 /* Launch configuration of the Jacobi-BiCGSTAB on this context (no counterpart in the
-   reference: diagnostics of the device solver): out[0] = SpMV form (3: one row per thread,
-   2: 128-row tiles, 1: 64-row tiles, 0: 64-row tiles with every slot's loads in flight),
-   out[1] = SpMV blocks, out[2] = vector-kernel blocks. */
+   reference: diagnostics of the device solver): out[0] = SpMV form (4: one row per thread
+   over XCD-local row ranges, 3: one row per thread, 2: 128-row tiles, 1: 64-row tiles, 0:
+   64-row tiles with every slot's loads in flight), out[1] = SpMV blocks, out[2] =
+   vector-kernel blocks. */
 int cpfem_krylov_config(const cpfem_ctx* ctx, int32_t* out);
 /* Forces the SpMV form of this context's BiCGSTAB (-1: picked by system size, the default).
    For tests and measurements that run two forms on one system; form 0 is refused
-   (CPFEM_EBADARG) on systems of 2^28 rows or more, form 3 on meshes whose rows have other
-   than 27 node slots.  No counterpart in the reference. */
+   (CPFEM_EBADARG) on systems of 2^28 rows or more, forms 3 and 4 on meshes whose rows have
+   other than 27 node slots.  No counterpart in the reference. */
 int cpfem_set_spmv_form(cpfem_ctx* ctx, int32_t form);
 /* Computed node columns of the one-row-per-thread SpMV on a box-numbered pattern (see
    cpfem_box_numbering): 1 = on (the default), 0 = every column loaded.  The products are

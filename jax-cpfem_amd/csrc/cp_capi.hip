@@ -576,7 +576,7 @@ static int run_bicgstab(cpfem_ctx* c, hipStream_t st, const KBufs& kb, const TM*
   w.sf = c->krylov_form();
   w.nbs = krylov_spmv_blocks(c->n_dofs, w.sf);
   w.ps = std::max(w.nb, w.nbs);
-  w.single = (w.sf == 3 && std::is_same<TM, double>::value) ? 1 : 0;
+  w.single = (w.sf >= 3 && std::is_same<TM, double>::value) ? 1 : 0;
   w.partial = kb.partial;
   w.sc = kb.sc;
   KVec v = kvec(const_cast<double**>(kb.v), x);
@@ -1043,9 +1043,9 @@ int cpfem_krylov_config(const cpfem_ctx* c, int32_t* out) {
 
 int cpfem_set_spmv_form(cpfem_ctx* c, int32_t form) {
   if (!c) return fail(CPFEM_EBADARG, "null context");
-  if (form < -1 || form > 3) return fail(CPFEM_EBADARG, "SpMV form must be -1 (by size), 0, 1, 2 or 3");
+  if (form < -1 || form > 4) return fail(CPFEM_EBADARG, "SpMV form must be -1 (by size), 0, 1, 2, 3 or 4");
   if (form >= 0 && !krylov_form_allowed(c->n_dofs, form, c->n_slots))
-    return fail(CPFEM_EBADARG, form == 3 ? "the one-row-per-thread SpMV form needs a 27-slot (hex8 box-like) mesh"
+    return fail(CPFEM_EBADARG, form >= 3 ? "the one-row-per-thread SpMV forms need a 27-slot (hex8 box-like) mesh"
                                          : "the all-slots SpMV form needs fewer than 2^28 rows");
   c->spmv_form = form;
   return CPFEM_OK;

@@ -189,10 +189,11 @@ struct KrylovWork {
   int64_t n;
   int nb;            // blocks of the vector kernels (one row per thread, grid-stride)
   int nbs;           // blocks of the SpMV kernels (one or two 64-row groups x 4 slot parts per block)
-  int sf;            // SpMV form (krylov_spmv_form): 3 = one row per thread, 2 = 128-row tiles,
-                     // 1 = 64-row tiles, 0 = 64-row tiles with every slot's loads in flight at once
+  int sf;            // SpMV form (krylov_spmv_form): 4 = one row per thread over XCD-local row
+                     // ranges, 3 = one row per thread, 2 = 128-row tiles, 1 = 64-row tiles,
+                     // 0 = 64-row tiles with every slot's loads in flight at once
   int ps;            // partial-sum stride: max(nb, nbs); unused entries hold zeros
-  int single;        // one-operand recurrence (FP64 flat form): phat formed by the x/r update
+  int single;        // one-operand recurrence (FP64 flat forms): phat formed by the x/r update
   double* partial;   // [slot][ps]
   KrylovScalars* sc;
 };

@@ -757,6 +757,14 @@ k_kst_all(BellView M, const TM* __restrict__ vals, KVec v, KrylovWork w, int par
 // release fence per block, an L2 write-back on gfx950: 286 us per SpMV at 64^3.)
 constexpr int FBS = 256;
 
+// First row of this block in the flat forms: form 3 takes row blocks in launch order; form 4
+// gives every XCD one contiguous range of row blocks (xcd_rank, as the tile forms' runs), so
+// its L2 holds one slab of the gathered vector and its halo.  Partial sums stay at the launch
+// index.
+__device__ __forceinline__ int64_t flat_row0(const KrylovWork& w) {
+  return (int64_t)(w.sf == 4 ? xcd_rank((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x) * FBS;
+}
+
 // Node columns of row r.  With a box numbering (M.box_b > 0) a node off the box's faces
 // computes them (i + dz A + dy B + dx, the ascending order its stored slots have), so only
 // face nodes load theirs: 27 x 4 bytes per node and the loads' round trip before the operand
@@ -854,7 +862,7 @@ __global__ void __launch_bounds__(FBS) k_kpq_flat(BellView M, const TM* __restri
   __shared__ double sh[2 * (FBS / 64)];
   KrylovScalars* sc = w.sc;
   const int64_t n = w.n;
-  const int64_t r = (int64_t)blockIdx.x * FBS + threadIdx.x;
+  const int64_t r = flat_row0(w) + threadIdx.x;
   int32_t j[27];
   const int done = sc->done;
   if (r < n) flat_cols(M, r, j);
@@ -908,7 +916,7 @@ __global__ void __launch_bounds__(FBS) k_kst_flat(BellView M, const TM* __restri
   __shared__ double sh[3 * (FBS / 64)];
   KrylovScalars* sc = w.sc;
   const int64_t n = w.n;
-  const int64_t r = (int64_t)blockIdx.x * FBS + threadIdx.x;
+  const int64_t r = flat_row0(w) + threadIdx.x;
   int32_t j[27];
   const int done = sc->done;
   if (r < n) flat_cols(M, r, j);
@@ -957,7 +965,7 @@ __global__ void __launch_bounds__(FBS) k_kpq_flat1(BellView M, const double* __r
   __shared__ double sh[2 * (FBS / 64)];
   KrylovScalars* sc = w.sc;
   const int64_t n = w.n;
-  const int64_t r = (int64_t)blockIdx.x * FBS + threadIdx.x;
+  const int64_t r = flat_row0(w) + threadIdx.x;
   int32_t j[27];
   const int done = sc->done;
   if (r < n) flat_cols(M, r, j);
@@ -1010,7 +1018,7 @@ __global__ void __launch_bounds__(FBS) k_kst_flat1(BellView M, const double* __r
   __shared__ double sh[5 * (FBS / 64)];
   KrylovScalars* sc = w.sc;
   const int64_t n = w.n;
-  const int64_t r = (int64_t)blockIdx.x * FBS + threadIdx.x;
+  const int64_t r = flat_row0(w) + threadIdx.x;
   int32_t j[27];
   const int done = sc->done;
   if (r < n) flat_cols(M, r, j);
@@ -1223,17 +1231,22 @@ static int device_cus() {
   return cus[dev];
 }
 
-// SpMV form picked by size: 3 = one row per thread (27-slot meshes with at least
-// FLAT_MIN_ROWS rows), 2 = 128-row tiles (at least two per CU), 0 = 64-row tiles with all slot
-// loads in flight (all tiles resident at three 256-thread blocks per CU), 1 = 64-row tiles
-// otherwise.  Measured on the dual-phase load step (s per load step, FP64 / mixed, form 3
-// against form 2): 32^3 0.407 / 0.360 against 0.415 / 0.358, 40^3 0.888 / 0.740 against
-// 0.885 / 0.735, 48^3 1.861 / 1.447 against 1.710 / 1.308, 56^3 3.257 / 2.268 against
-// 3.311 / 2.146, 64^3 (823,875 rows) 5.126 / 3.936 against 5.328 / 3.989.
+// SpMV form picked by size: on 27-slot meshes 3 = one row per thread (at least FLAT_MIN_ROWS
+// rows) and 4 = one row per thread over XCD-local row ranges (from FLAT_XCD_MIN_ROWS rows);
+// otherwise 2 = 128-row tiles (at least two per CU), 0 = 64-row tiles with all slot loads in
+// flight (all tiles resident at three 256-thread blocks per CU), 1 = 64-row tiles.  Measured
+// on the dual-phase load step (s per load step, FP64 / mixed; form 2 against 3 against 4):
+// 32^3 (107,811 rows) 0.384 / 0.335, 0.372 / 0.334, 0.366 / 0.336; 40^3 0.809 / 0.681, 0.808
+// / 0.701, 0.767 / 0.650; 48^3 1.557 / 1.184, 1.677 / 1.307, 1.384 / 1.072; 56^3 3.138 /
+// 1.989, 3.002 / 1.975, 3.030 / 1.784; 64^3 (823,875 rows) forms 3 and 4: 4.638 / 3.513
+// against 4.792 / 3.704.  (The XCD-local ranges keep each XCD's gathers in its own L2 slab;
+// at 64^3 the plain launch order streams better.)
 constexpr int64_t FLAT_MIN_ROWS = 700000;
+constexpr int64_t FLAT_XCD_MIN_ROWS = 100000;
 int krylov_spmv_form(int64_t n, int n_slots) {
   const int cus = device_cus();
   if (n_slots == 27 && n >= FLAT_MIN_ROWS) return 3;
+  if (n_slots == 27 && n >= FLAT_XCD_MIN_ROWS) return 4;
   if ((n + SpG<2>::BLOCK_ROWS - 1) / SpG<2>::BLOCK_ROWS >= 2 * cus) return 2;
   return (n + SpG<1>::BLOCK_ROWS - 1) / SpG<1>::BLOCK_ROWS <= 3 * cus ? 0 : 1;
 }
@@ -1245,7 +1258,7 @@ int krylov_spmv_form(int64_t n, int n_slots) {
 // matrix with 1024-thread blocks, 256 blocks took 46.3 us per BiCGSTAB iteration, 384 blocks
 // 61.7 us, 422 blocks 50.4 us).
 int krylov_spmv_blocks(int64_t n, int form) {
-  if (form == 3) return (int)std::max<int64_t>(1, (n + FBS - 1) / FBS);
+  if (form == 3 || form == 4) return (int)std::max<int64_t>(1, (n + FBS - 1) / FBS);
   const int64_t per_cu = form == 2 ? 2 : form == 0 ? 3 : 4;
   const int64_t rows = form == 2 ? SpG<2>::BLOCK_ROWS : SpG<1>::BLOCK_ROWS;
   return (int)std::max<int64_t>(1, std::min<int64_t>((n + rows - 1) / rows, per_cu * device_cus()));
@@ -1257,7 +1270,7 @@ int krylov_spmv_blocks(int64_t n, int form) {
 // 148 us for the 128-row form.)
 bool krylov_form_allowed(int64_t n, int form, int n_slots) {
   if (form == 1 || form == 2) return true;
-  if (form == 3) return n_slots == 27 && (n + FBS - 1) / FBS <= INT32_MAX;
+  if (form == 3 || form == 4) return n_slots == 27 && (n + FBS - 1) / FBS <= INT32_MAX;
   return form == 0 && n < ((int64_t)1 << 28);
 }
 
@@ -1284,7 +1297,7 @@ static inline void timed_launch(Prof* prof, int cls, F kernel, dim3 grid, dim3 b
 template <typename TM>
 void krylov_iteration(const BellView& M, const TM* vals, KVec v, KrylovWork w, int par, hipStream_t st, Prof* prof) {
   const unsigned nb = (unsigned)w.nb, nbs = (unsigned)w.nbs;
-  if (w.sf == 3) {
+  if (w.sf >= 3) {
     if constexpr (std::is_same<TM, double>::value) {
       if (w.single) {
         timed_launch(prof, PC_SPMV, k_kpq_flat1, dim3(nbs), dim3(FBS), st, M, vals, v, w, par);

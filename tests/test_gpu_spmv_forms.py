@@ -1,9 +1,11 @@
-"""The forms of the BiCGSTAB SpMV kernels.  Three are tested at a size that selects them on a
+"""The forms of the BiCGSTAB SpMV kernels.  Four are tested at a size that selects them on a
 256-CU device: 64-row tiles with every slot's loads in flight (16^3, 14,739 dofs: 231 tiles),
 64-row tiles one slot at a time (26^3, 59,049 dofs: 923 tiles, above three per CU), 128-row
-tiles (28^3, 73,167 dofs: 572 tiles, two per CU); the fourth, one row per thread, is picked
-from 700,000 rows (the 64^3 benchmark system, covered by test_gpu_fullsize.py) and is forced
-here on small systems.  The oracle is too slow at these sizes, so each
+tiles (28^3, 73,167 dofs: 572 tiles, two per CU), one row per thread over XCD-local row
+ranges (32^3, 107,811 dofs; picked on 27-slot meshes from 100,000 rows); the fifth, one row
+per thread in launch order, is picked from 700,000 rows (the 64^3 benchmark system, covered
+by test_gpu_fullsize.py).  The two one-row-per-thread forms are also forced here on small
+systems.  The oracle is too slow at these sizes, so each
 case checks what fixes the answer independently of the device kernels, on the first Newton
 system of the 304-like polycrystal (elastic tangent, Dirichlet values in x0), with the true
 residual computed by scipy from the exported CSR.  The recurrence stops at
@@ -43,7 +45,7 @@ def _first_system(n):
     return prob, mat, -res, bc_x0(prob, dofs)
 
 
-@pytest.mark.parametrize("n,form", [(16, 0), (26, 1), (28, 2)])
+@pytest.mark.parametrize("n,form", [(16, 0), (26, 1), (28, 2), (32, 4)])
 def test_spmv_form_solves_first_newton_system(n, form):
     import cpfem_amd as C
     from cpfem_amd.solver import krylov_solve
@@ -114,9 +116,11 @@ def test_all_slots_form_reproduces_one_slot_form_bitwise():
 
 
 @pytest.mark.parametrize("n", [4, 16, 28])
-def test_flat_form_solves_first_newton_system(n):
-    """The one-row-per-thread form (3), forced: one partial sum per 256-row block, added by the
-    consuming kernel, so its dot products run in a different order from the tile forms'.
+@pytest.mark.parametrize("flat", [3, 4])
+def test_flat_form_solves_first_newton_system(n, flat):
+    """The one-row-per-thread forms (3: row blocks in launch order, 4: XCD-local row ranges),
+    forced: one partial sum per 256-row block, added by the consuming kernel, so their dot
+    products run in a different order from the tile forms'.
     FP64 and mixed solutions meet the same true-residual bar as the size-picked form and land
     within 1e-6 of it.  Iteration counts are not compared: BiCGSTAB's count moves with the
     summation order by up to ~10% on these systems in FP64 (28^3: 353 against 319) and more in
@@ -133,13 +137,13 @@ def test_flat_form_solves_first_newton_system(n):
         N.check(prob.ctx._lib.cpfem_set_spmv_form(prob.ctx.h, -1))
         xa = krylov_solve(prob, mat, b, x0, dict(precision=prec)).cpu().numpy()
         ka = prob.solver_info[-1]["iterations"]
-        N.check(prob.ctx._lib.cpfem_set_spmv_form(prob.ctx.h, 3))
-        assert _form(prob) == 3
+        N.check(prob.ctx._lib.cpfem_set_spmv_form(prob.ctx.h, flat))
+        assert _form(prob) == flat
         xf = krylov_solve(prob, mat, b, x0, dict(precision=prec)).cpu().numpy()
         kf = prob.solver_info[-1]["iterations"]
         res = np.linalg.norm(A @ xf - bh)
         err = np.linalg.norm(xf - xa) / np.linalg.norm(xa)
-        print(f"n={n} {prec}: picked form {ka} iterations, flat form {kf}, |Ax - b| {res:.2e} (bar {bar:.2e}), "
+        print(f"n={n} {prec}: picked form {ka} iterations, flat form {flat} {kf}, |Ax - b| {res:.2e} (bar {bar:.2e}), "
               f"|x - x_picked|/|x| {err:.2e}", flush=True)
         assert res <= bar, (prec, res, bar)
         assert err < 1e-6, (prec, err)

@@ -127,3 +127,5 @@ def test_measured_traffic_table():
     assert bench.spmv_kernel_names(0, "double")[1] == "cpfem::k_kst_all<double>"
     assert bench.spmv_kernel_names(3, "float") == ["cpfem::k_kpq_flat<float>", "cpfem::k_kst_flat<float>"]
     assert bench.spmv_kernel_names(3, "double") == ["cpfem::k_kpq_flat1", "cpfem::k_kst_flat1"]
+    assert bench.spmv_kernel_names(4, "double") == ["cpfem::k_kpq_flat1", "cpfem::k_kst_flat1"]
+    assert bench.spmv_kernel_names(4, "float") == ["cpfem::k_kpq_flat<float>", "cpfem::k_kst_flat<float>"]
