@@ -1240,7 +1240,9 @@ static int device_cus() {
 // / 0.701, 0.767 / 0.650; 48^3 1.557 / 1.184, 1.677 / 1.307, 1.384 / 1.072; 56^3 3.138 /
 // 1.989, 3.002 / 1.975, 3.030 / 1.784; 64^3 (823,875 rows) forms 3 and 4: 4.638 / 3.513
 // against 4.792 / 3.704.  (The XCD-local ranges keep each XCD's gathers in its own L2 slab;
-// at 64^3 the plain launch order streams better.)
+// at 64^3 the plain launch order streams better.)  Below 100,000 rows the tile forms stay:
+// 16^3 0.105 / 0.117 with form 0 against 0.130 / 0.130 with form 4, 24^3 0.184 / 0.199
+// against 0.198 / 0.197; 28^3 (73,167 rows) 0.302 / 0.279 with form 2 against 0.291 / 0.268.
 constexpr int64_t FLAT_MIN_ROWS = 700000;
 constexpr int64_t FLAT_XCD_MIN_ROWS = 100000;
 int krylov_spmv_form(int64_t n, int n_slots) {
