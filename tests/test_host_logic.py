@@ -117,10 +117,13 @@ def test_measured_traffic_table():
     spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(root, "bench.py"))
     bench = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(bench)
-    t = bench.measured_traffic("poly32", "fp64", ["cpfem::k_kpq<double, 2>", "cpfem::k_kst<double, 2>"])
+    t = bench.measured_traffic("c2_32", "fp64", ["cpfem::k_kpq_flat1", "cpfem::k_kst_flat1"])
     assert t is not None and 60e6 < t < 120e6
-    assert bench.measured_traffic("poly32", "fp64", ["cpfem::no_such_kernel"]) is None
-    assert bench.measured_traffic("nowhere", "fp64", ["cpfem::k_kpq<double, 2>"]) is None
+    assert bench.measured_traffic("c2_32", "fp64", ["cpfem::no_such_kernel"]) is None
+    assert bench.measured_traffic("c2_32", "mixed", ["cpfem::k_kpq_flat1", "cpfem::k_kst_flat1"]) is None
+    assert bench.measured_traffic("nowhere", "fp64", ["cpfem::k_kpq_flat1"]) is None
+    t64 = bench.measured_traffic("c5_64", "fp64", ["cpfem::k_kpq_flat1", "cpfem::k_kst_flat1"])
+    assert t64 is not None and 600e6 < t64 < 800e6
     # kernel names per SpMV form (the form itself is the library's report, cpfem_krylov_config)
     assert bench.spmv_kernel_names(2, "double") == ["cpfem::k_kpq<double, 2>", "cpfem::k_kst<double, 2>"]
     assert bench.spmv_kernel_names(1, "float") == ["cpfem::k_kpq<float, 1>", "cpfem::k_kst<float, 1>"]
