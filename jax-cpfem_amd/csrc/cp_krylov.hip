@@ -760,15 +760,16 @@ constexpr int FBS = 256;
 // First row of this block in the flat forms: form 3 takes row blocks in launch order; form 4
 // gives every XCD one contiguous range of row blocks (xcd_rank, as the tile forms' runs), so
 // its L2 holds one slab of the gathered vector and its halo.  Partial sums stay at the launch
-// index.  rev (the second SpMV kernel of form 3): row blocks in reverse launch order, so that
-// each SpMV starts on the rows the previous one read last.  The matrix streams twice per
-// iteration and at 64^3 is larger than the 256 MB Infinity Cache (FP64 534 MB, FP32 267 MB):
-// walking it back and forth, each pass finds the other's last rows still in the cache (the
-// SpMV pair at 64^3: 133.8 -> 122.0 us per launch in FP64, 82.2 -> 75.8 us with FP32
-// values).  At the form-4 sizes it changed nothing measurable.
+// index.  rev (the second SpMV kernel): the same row blocks in reverse order, so that each
+// SpMV starts on the rows the previous one read last.  The matrix streams twice per
+// iteration and from 56^3 up it is larger than the 256 MB Infinity Cache (64^3: FP64 534 MB,
+// FP32 267 MB): walked back and forth, each pass finds the other's last rows still in the
+// cache (the SpMV pair at 64^3: 133.8 -> 122.0 us per launch in FP64, 82.2 -> 75.8 us with
+// FP32 values; at 56^3, form 4: 92.5 -> 80.3 us in FP64).  Where the matrix fits the cache
+// (48^3) it changes nothing measurable.
 __device__ __forceinline__ int64_t flat_row0(const KrylovWork& w, bool rev = false) {
-  if (w.sf == 4) return (int64_t)xcd_rank((int)blockIdx.x, (int)gridDim.x) * FBS;
-  return (int64_t)(rev ? (int)(gridDim.x - 1 - blockIdx.x) : (int)blockIdx.x) * FBS;
+  const int b = w.sf == 4 ? xcd_rank((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+  return (int64_t)(rev ? (int)gridDim.x - 1 - b : b) * FBS;
 }
 
 // Node columns of row r.  With a box numbering (M.box_b > 0) a node off the box's faces
@@ -1250,7 +1251,9 @@ static int device_cus() {
 // / 0.701, 0.767 / 0.650; 48^3 1.557 / 1.184, 1.677 / 1.307, 1.384 / 1.072; 56^3 3.138 /
 // 1.989, 3.002 / 1.975, 3.030 / 1.784; 64^3 (823,875 rows) forms 3 and 4: 4.638 / 3.513
 // against 4.792 / 3.704.  (The XCD-local ranges keep each XCD's gathers in its own L2 slab;
-// at 64^3 the plain launch order streams better.)  Below 100,000 rows the tile forms stay:
+// at 64^3 the plain launch order streams better.  These are one-direction figures: with the
+// second SpMV walking back, form 4 at 56^3 takes 2.724 / 1.720 s.)  Below 100,000 rows the
+// tile forms stay:
 // 16^3 0.105 / 0.117 with form 0 against 0.130 / 0.130 with form 4, 24^3 0.184 / 0.199
 // against 0.198 / 0.197; 28^3 (73,167 rows) 0.302 / 0.279 with form 2 against 0.291 / 0.268.
 constexpr int64_t FLAT_MIN_ROWS = 700000;
