@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 
-@pytest.mark.parametrize("form", [-1, 0, 1, 3])
+@pytest.mark.parametrize("form", [-1, 0, 1, 3, 4])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_krylov_solve_repeats_bitwise(form, precision):
     from test_gpu_spmv_forms import _first_system
