@@ -138,7 +138,6 @@ struct cpfem_ctx {
   int nb = 0;
   double* d_partial = nullptr;
   double* d_scalar = nullptr;
-  uint32_t* d_counter = nullptr;
   // Krylov
   KrylovScalars* d_ksc = nullptr;
   double* d_kpartial = nullptr;  // [PS_COUNT][krylov blocks]
@@ -477,7 +476,7 @@ static int ensure_krylov(cpfem_ctx* c, bool mixed) {
 
 // |a . b| finished on the device, copied to the host (synchronising)
 static int dot_host(cpfem_ctx* c, hipStream_t st, const double* a, const double* b, double* out) {
-  launch_residual_norm(c->n_dofs, a, b, c->d_partial, c->nb, c->d_scalar, c->d_counter, st);
+  launch_residual_norm(c->n_dofs, a, b, c->d_partial, c->nb, c->d_scalar, st);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(c->h_dbl, c->d_scalar, sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
@@ -717,15 +716,14 @@ int cpfem_create(const CpfemMesh* mesh, const CpfemSlip* slip, cpfem_ctx** out) 
   if ((rc = dalloc(&c->d_isbc, c->n_dofs)) || (rc = dalloc(&c->d_nobc, c->n_dofs)) || (rc = dalloc(&c->d_ubar, c->n_dofs)) ||
       (rc = dalloc(&c->d_wP, (size_t)9 * c->n_qp)) || (rc = dalloc(&c->d_RE, (size_t)24 * c->n_cells)) ||
       (rc = dalloc(&c->d_sig, (size_t)9 * c->n_qp)) || (rc = dalloc(&c->d_status, 1)) || (rc = dalloc(&c->d_counts, 3)) ||
-      (rc = dalloc(&c->d_partial, (size_t)4 * std::max(c->nb, nbq))) || (rc = dalloc(&c->d_scalar, 4)) ||
-      (rc = dalloc(&c->d_counter, 4))) {
+      (rc = dalloc(&c->d_partial, (size_t)4 * std::max(c->nb, nbq))) || (rc = dalloc(&c->d_scalar, 4))) {
     const std::string e = g_err;
     cpfem_destroy(c);
     g_err = e;
     return rc;
   }
   if (hipMemset(c->d_isbc, 0, c->n_dofs) != hipSuccess || hipMemset(c->d_nobc, 0, c->n_dofs) != hipSuccess || hipMemset(c->d_ubar, 0, 8 * c->n_dofs) != hipSuccess ||
-      hipMemset(c->d_status, 0, 4) != hipSuccess || hipMemset(c->d_counter, 0, 16) != hipSuccess ||
+      hipMemset(c->d_status, 0, 4) != hipSuccess ||
       hipMemset(c->d_counts, 0, 24) != hipSuccess ||
       hipHostMalloc((void**)&c->h_ksc, sizeof(KrylovScalars)) != hipSuccess ||
       hipHostMalloc((void**)&c->h_dbl, 8 * sizeof(double)) != hipSuccess ||
@@ -751,7 +749,7 @@ void cpfem_destroy(cpfem_ctx* c) {
   dfree(c->d_gi); dfree(c->d_sconv);
   dfree(c->d_wP); dfree(c->d_wA); dfree(c->d_KE); dfree(c->d_RE); dfree(c->d_sig); dfree(c->d_status); dfree(c->d_counts);
   dfree(c->d_park_idx); dfree(c->d_park_it); dfree(c->d_park_s); dfree(c->d_park_ctr);
-  dfree(c->d_partial); dfree(c->d_scalar); dfree(c->d_counter);
+  dfree(c->d_partial); dfree(c->d_scalar);
   dfree(c->d_ksc);
   dfree(c->d_kpartial);
   for (int i = 0; i < 12; ++i) dfree(c->kv64[i]);

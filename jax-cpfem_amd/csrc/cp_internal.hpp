@@ -241,7 +241,7 @@ void krylov_replace(const BellView& M, const double* vals64, const double* b, KV
 void launch_spmv(const BellView& M, const double* vals, const double* x, double* y, hipStream_t st);
 void launch_inv_diag(const BellView& M, const double* vals, const int32_t* dslot, double* invd, int precond, hipStream_t st);
 
-void launch_residual_norm(int64_t n, const double* a, const double* b, double* partial, int nb, double* out, uint32_t* counter,
+void launch_residual_norm(int64_t n, const double* a, const double* b, double* partial, int nb, double* out,
                           hipStream_t st);
 void launch_axpby(int64_t n, double alpha, const double* x, double beta, double* y, hipStream_t st);
 void launch_cast_f32(int64_t n, const double* x, float* y, hipStream_t st);

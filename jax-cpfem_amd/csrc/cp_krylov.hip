@@ -85,40 +85,6 @@ __device__ __forceinline__ void block_sums(double* v, double* sh) {
   __syncthreads();
 }
 
-// Publishes this block's NS partial sums; returns true in every thread of the last block,
-// which then holds the totals in tot[] (thread 0 only).
-template <int NS>
-__device__ __forceinline__ bool finish_sums(double* v, double* partial, int nb, uint32_t* counter, double* tot) {
-  __shared__ double sh[NS * (KBS / 64)];
-  __shared__ int last;
-  block_sums<NS>(v, sh);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int j = 0; j < NS; ++j) partial[j * nb + blockIdx.x] = v[j];
-    __threadfence();
-    const uint32_t prev = atomicAdd(counter, 1u);
-    last = (prev == (uint32_t)(gridDim.x - 1));
-  }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  double acc[NS];
-  const volatile double* vp = partial;
-#pragma unroll
-  for (int j = 0; j < NS; ++j) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < nb; i += KBS) s += vp[j * nb + i];
-    acc[j] = s;
-  }
-  block_sums<NS>(acc, sh);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int j = 0; j < NS; ++j) tot[j] = acc[j];
-    *counter = 0u;
-  }
-  return true;
-}
-
 // Row r of A x.  Hex meshes have exactly 27 node slots (padding slots hold zeros and point
 // at the row's own node), so the common case runs a fixed, fully unrolled slot loop: all
 // column indices are loaded first, then all values and x entries, giving 27-way memory-level
@@ -1193,13 +1159,24 @@ __global__ void __launch_bounds__(KBS) k_kreplace(BellView M, const double* __re
   put_partials<KBS, 2>({rr, rhr}, {PS_RR, PS_RHR}, w.partial, w.ps, sh);
 }
 
-// Generic dot product a.b finished on device into *out
+// Generic dot product a.b into *out: k_dot writes one partial per block, k_dot_finish (one
+// block) adds them in index order.  (Finishing in k_dot's last block needs an agent-scope
+// fence in every block, an L2 write-back on gfx950: 68 us for one 64^3 vector, ~10 us this
+// way.)
 __global__ void __launch_bounds__(KBS) k_dot(int64_t n, const double* __restrict__ a, const double* __restrict__ b,
-                                            double* partial, int nb, double* out, uint32_t* counter) {
+                                            double* __restrict__ partial) {
+  __shared__ double sh[KBS / 64];
   const int64_t i = (int64_t)blockIdx.x * KBS + threadIdx.x;
   double d[1] = {i < n ? a[i] * b[i] : 0.0};
-  double tot[1];
-  if (finish_sums<1>(d, partial, nb, counter, tot) && threadIdx.x == 0) *out = tot[0];
+  block_sums<1>(d, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = d[0];
+}
+__global__ void __launch_bounds__(KBS) k_dot_finish(const double* __restrict__ partial, int nb, double* __restrict__ out) {
+  __shared__ double sh[KBS / 64];
+  double acc[1] = {0.0};
+  for (int i = threadIdx.x; i < nb; i += KBS) acc[0] += partial[i];
+  block_sums<1>(acc, sh);
+  if (threadIdx.x == 0) *out = acc[0];
 }
 
 __global__ void __launch_bounds__(KBS) k_copy(int64_t n, const double* __restrict__ x, double* __restrict__ y) {
@@ -1344,9 +1321,10 @@ void launch_inv_diag(const BellView& M, const double* vals, const int32_t* dslot
   hipLaunchKernelGGL(k_inv_diag, dim3(nblk(3 * M.n_nodes)), dim3(KBS), 0, st, M, vals, dslot, invd, precond);
 }
 
-void launch_residual_norm(int64_t n, const double* a, const double* b, double* partial, int nb, double* out, uint32_t* counter,
+void launch_residual_norm(int64_t n, const double* a, const double* b, double* partial, int nb, double* out,
                           hipStream_t st) {
-  hipLaunchKernelGGL(k_dot, dim3((unsigned)nb), dim3(KBS), 0, st, n, a, b, partial, nb, out, counter);
+  hipLaunchKernelGGL(k_dot, dim3((unsigned)nb), dim3(KBS), 0, st, n, a, b, partial);
+  hipLaunchKernelGGL(k_dot_finish, dim3(1), dim3(KBS), 0, st, partial, nb, out);
 }
 void launch_copy(int64_t n, const double* x, double* y, hipStream_t st) {
   hipLaunchKernelGGL(k_copy, dim3(nblk(n)), dim3(KBS), 0, st, n, x, y);
