@@ -1161,7 +1161,8 @@ __global__ void __launch_bounds__(KBS) k_kreplace(BellView M, const double* __re
 
 // Generic dot product a.b into *out: k_dot writes one partial per block, k_dot_finish (one
 // block) adds them in index order.  (Finishing in k_dot's last block needs an agent-scope
-// fence in every block, an L2 write-back on gfx950: 68 us for one 64^3 vector.)
+// fence in every block, an L2 write-back on gfx950: 68 us for one 64^3 vector, against
+// 4.6 + 4.7 us for the two kernels.)
 __global__ void __launch_bounds__(KBS) k_dot(int64_t n, const double* __restrict__ a, const double* __restrict__ b,
                                             double* __restrict__ partial) {
   __shared__ double sh[KBS / 64];
